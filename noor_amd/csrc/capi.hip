@@ -1,0 +1,566 @@
+// C ABI of the render path (include/noor_hip.h).
+//
+// Replaces the reference's load_cuda_data / update_cuda_* / cuda_path_tracer /
+// device_free_memory / get_lookAt (noor/cuda/path_tracer.cuh:136-151,
+// noor/cuda/path_tracer_data.cuh:51-84, noor/cuda/render.cuh:37-265):
+//   - the payload is validated and deep-copied to HBM; nothing is retained;
+//   - the __constant__ manager singletons become one DevScene passed by value;
+//   - CUDA texture objects become raw pointers with software filtering;
+//   - every failure returns a status and records a message instead of exit().
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+
+using noor::Context;
+
+struct noor_ctx {
+    Context c;
+};
+
+namespace {
+
+const char* kStatus[] = {"ok", "invalid argument", "invalid payload", "HIP error", "out of memory",
+                         "stack depth exceeded", "no device", "unsupported", "queue overflow"};
+
+noor_status fail(Context* c, noor_status s, const std::string& msg) {
+    if (c) c->last_error = msg;
+    return s;
+}
+
+#define HIP_TRY(c, expr)                                                                           \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            return fail((c), e_ == hipErrorOutOfMemory ? NOOR_ERR_OUT_OF_MEMORY : NOOR_ERR_HIP,    \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+        }                                                                                          \
+    } while (0)
+
+template <class T>
+noor_status dev_upload(Context* c, T** dst, const T* src, size_t count) {
+    *dst = nullptr;
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    HIP_TRY(c, hipMalloc(&p, sizeof(T) * count));
+    c->allocs.push_back(p);
+    if (src) HIP_TRY(c, hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+    else HIP_TRY(c, hipMemsetAsync(p, 0, sizeof(T) * count, c->stream));
+    *dst = (T*)p;
+    return NOOR_OK;
+}
+
+uint32_t stack_need(const noor_payload_view* P, uint32_t ni, std::vector<int32_t>& memo, int depth, bool* bad) {
+    if (ni >= P->num_bvh_nodes || depth > 4096) {
+        *bad = true;
+        return 0;
+    }
+    if (memo[ni] >= 0) return (uint32_t)memo[ni];
+    const noor_uint4 a = P->bvh_min_right_start[ni];
+    const uint32_t w = P->bvh_max_axis_count[ni].w;
+    uint32_t need = 0;
+    if (w & NOOR_LIGHT_NODE_MASK) {
+        need = 0;
+    } else if (w & NOOR_MESH_INSTANCE_MASK) {
+        need = std::max(1u, stack_need(P, a.w, memo, depth + 1, bad));
+    } else if (w & NOOR_MESH_LEAF_MASK) {
+        const uint32_t k = w & NOOR_COUNT_MASK;
+        need = k;
+        for (uint32_t j = 0; j < k && !*bad; ++j)
+            need = std::max(need, (k - 1 - j) + stack_need(P, ni + 1 + j, memo, depth + 1, bad));
+    } else if (w & NOOR_TRI_LEAF_MASK) {
+        need = 0;
+    } else {
+        const uint32_t l = stack_need(P, ni + 1, memo, depth + 1, bad);
+        const uint32_t r = stack_need(P, a.w, memo, depth + 1, bad);
+        need = std::max(2u, 1u + std::max(l, r));
+    }
+    memo[ni] = (int32_t)need;
+    return need;
+}
+
+const uint32_t kSupportedMaterials = NOOR_MAT_DIFFUSE | NOOR_MAT_TRANSLUCENT | NOOR_MAT_GLOSSY | NOOR_MAT_MIRROR |
+                                     NOOR_MAT_GLASS | NOOR_MAT_METAL | NOOR_MAT_EMITTER;
+
+noor_status validate(Context* c, const noor_payload_view* P) {
+    if (P->num_triangles && (!P->attribute_indices || !P->wald_ax_nu_nv_nd || !P->wald_bnu_bnv_au_av ||
+                             !P->wald_cnu_cnv_deg_mat))
+        return fail(c, NOOR_ERR_INVALID_ARG, "triangle arrays missing");
+    if (!P->bvh_min_right_start || !P->bvh_max_axis_count || P->num_bvh_nodes == 0)
+        return fail(c, NOOR_ERR_INVALID_ARG, "BVH arrays missing");
+    if (P->bvh_root_node >= P->num_bvh_nodes) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "root out of range");
+    if (P->num_vertices && (!P->vertices || !P->normals || !P->uvs))
+        return fail(c, NOOR_ERR_INVALID_ARG, "vertex arrays missing");
+    if (P->num_materials == 0 || !P->materials) return fail(c, NOOR_ERR_INVALID_ARG, "no materials");
+    if (P->num_textures == 0 || !P->textures) return fail(c, NOOR_ERR_INVALID_ARG, "no textures");
+    for (uint32_t i = 0; i < P->num_vertices; ++i) {
+        const noor_float4 v = P->vertices[i];
+        if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z))
+            return fail(c, NOOR_ERR_INVALID_PAYLOAD, "non-finite vertex " + std::to_string(i));
+    }
+    for (uint32_t t = 0; t < P->num_triangles; ++t) {
+        const noor_uint4 a = P->attribute_indices[t];
+        if (a.x >= P->num_vertices || a.y >= P->num_vertices || a.z >= P->num_vertices)
+            return fail(c, NOOR_ERR_INVALID_PAYLOAD, "vertex index out of range in triangle " + std::to_string(t));
+        if (a.w >= P->num_materials)
+            return fail(c, NOOR_ERR_INVALID_PAYLOAD, "material index out of range in triangle " + std::to_string(t));
+        if (P->wald_ax_nu_nv_nd[t].x > 2u) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "bad Wald axis");
+    }
+    for (uint32_t n = 0; n < P->num_bvh_nodes; ++n) {
+        const noor_uint4 a = P->bvh_min_right_start[n];
+        const uint32_t w = P->bvh_max_axis_count[n].w;
+        if (w & NOOR_LIGHT_NODE_MASK) {
+            if ((w & NOOR_LIGHT_MASK) >= P->num_area_lights)
+                return fail(c, NOOR_ERR_INVALID_PAYLOAD, "light node index out of range");
+        } else if (w & NOOR_MESH_INSTANCE_MASK) {
+            if ((w & NOOR_TRANS_MASK) >= P->num_instances || a.w >= P->num_bvh_nodes)
+                return fail(c, NOOR_ERR_INVALID_PAYLOAD, "instance node out of range");
+        } else if (w & NOOR_MESH_LEAF_MASK) {
+            if ((uint64_t)n + (w & NOOR_COUNT_MASK) >= P->num_bvh_nodes)
+                return fail(c, NOOR_ERR_INVALID_PAYLOAD, "mesh leaf overruns the node array");
+        } else if (w & NOOR_TRI_LEAF_MASK) {
+            if ((uint64_t)a.w + (w & NOOR_TRI_COUNT_MASK) > P->num_triangles)
+                return fail(c, NOOR_ERR_INVALID_PAYLOAD, "triangle leaf out of range");
+        } else {
+            if ((w & NOOR_AXIS_MASK) > 2u || a.w >= P->num_bvh_nodes || n + 1 >= P->num_bvh_nodes)
+                return fail(c, NOOR_ERR_INVALID_PAYLOAD, "inner node out of range");
+        }
+    }
+    for (uint32_t m = 0; m < P->num_materials; ++m) {
+        const noor_material& mt = P->materials[m];
+        if ((mt.type & ~kSupportedMaterials) != 0 || mt.type == 0)
+            return fail(c, NOOR_ERR_UNSUPPORTED, "material type " + std::to_string(mt.type) + " not supported");
+        const uint32_t idx[] = {mt.diffuse_tex_idx, mt.specular_tex_idx, mt.transmission_tex_idx,
+                                mt.emittance_tex_idx, mt.ior_tex_idx, mt.k_tex_idx, mt.roughness_tex_idx};
+        for (uint32_t t : idx) {
+            if (t >= P->num_textures) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "texture index out of range");
+            const noor_texture_view& tv = P->textures[t];
+            if (tv.width != 1 || tv.height != 1 || !tv.rgba)
+                return fail(c, NOOR_ERR_UNSUPPORTED, "only 1x1 material textures are supported");
+        }
+    }
+    const noor_texture_view& env = P->textures[P->num_textures - 1];
+    if (!env.rgba || env.width == 0 || env.height == 0) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "no environment texture");
+    if (P->num_infinite_lights > 1) return fail(c, NOOR_ERR_UNSUPPORTED, "more than one infinite light");
+    return NOOR_OK;
+}
+
+int nearest_pow2(int n) {
+    int next = 1;
+    while (next < n) next <<= 1;
+    int prev = next;
+    if (prev > n) prev >>= 1;
+    if (prev < 1) prev = 1;
+    return (std::abs(n - next) < std::abs(n - prev)) ? next : prev;
+}
+
+noor_status apply_spec(Context* c, const noor_spec* spec) {
+    c->spec = *spec;
+    c->S.sky_enabled = (spec->lighting_type & NOOR_LIGHT_SKY_MASK) ? 1 : 0;
+    c->S.skydome_type = spec->skydome_type;
+    c->S.bounces = spec->bounces;
+    c->S.rr = spec->rr;
+    c->S.mis = spec->mis ? 1 : 0;
+    c->S.bias = spec->reflection_bias;
+    c->S.world_radius = spec->world_radius;
+    if (spec->bvh_root_node != c->S.root) return fail(c, NOOR_ERR_INVALID_ARG, "spec root differs from payload root");
+    return NOOR_OK;
+}
+
+noor_status apply_options(Context* c, const noor_options* o) {
+    noor_options def{NOOR_RNG_PCG32, NOOR_TRI_WATERTIGHT, NOOR_COMPAT_ALL, 0};
+    c->opt = o ? *o : def;
+    if (c->opt.rng != NOOR_RNG_PCG32 && c->opt.rng != NOOR_RNG_TEA_LCG) return fail(c, NOOR_ERR_INVALID_ARG, "rng");
+    if (c->opt.tri_test != NOOR_TRI_WATERTIGHT && c->opt.tri_test != NOOR_TRI_WALD)
+        return fail(c, NOOR_ERR_INVALID_ARG, "tri_test");
+    c->S.rng_kind = c->opt.rng;
+    c->S.tri_test = c->opt.tri_test;
+    c->S.compat = c->opt.compat;
+    return NOOR_OK;
+}
+
+noor_status apply_camera(Context* c, const noor_camera* cam) {
+    if (cam->w == 0 || cam->h == 0 || cam->w > 16384 || cam->h > 16384)
+        return fail(c, NOOR_ERR_INVALID_ARG, "camera size");
+    if (cam->type < 0 || cam->type > 2) return fail(c, NOOR_ERR_INVALID_ARG, "camera type");
+    c->cam = *cam;
+    c->S.cam = *cam;
+    if (cam->w != c->film_w || cam->h != c->film_h) {
+        if (c->film) {
+            hipFree(c->film);
+            c->film = nullptr;
+        }
+        void* p = nullptr;
+        HIP_TRY(c, hipMalloc(&p, sizeof(float4) * (size_t)cam->w * cam->h));
+        HIP_TRY(c, hipMemsetAsync(p, 0, sizeof(float4) * (size_t)cam->w * cam->h, c->stream));
+        c->film = (float4*)p;
+        c->film_w = cam->w;
+        c->film_h = cam->h;
+    }
+    return NOOR_OK;
+}
+
+noor_status build_environment(Context* c) {
+    // PHYSICAL: bake sin(theta) * sky into a 2048^2 texture, then the CDF of it
+    // (noor/cuda/skydome.cuh:58-71); HDR: the CDF of the environment texture.
+    const float4* tex = c->S.env;
+    int tw = (int)c->S.env_w, th = (int)c->S.env_h;
+    if (c->S.skydome_type == NOOR_SKY_PHYSICAL) {
+        if (!c->sky_bake) {
+            void* p = nullptr;
+            HIP_TRY(c, hipMalloc(&p, sizeof(float4) * 2048 * 2048));
+            c->sky_bake = (float4*)p;
+        }
+        HIP_TRY(c, noor::launch_sky_bake(*c, c->sky_bake, 2048, 2048));
+        tex = c->sky_bake;
+        tw = th = 2048;
+    }
+    const int W = std::min(std::max(nearest_pow2(tw), 32), 1024);
+    const int H = std::min(std::max(nearest_pow2(th), 32), 1024);
+    const size_t n = (size_t)H * (W + 2) + 1;
+    if (c->dfunc) hipFree(c->dfunc);
+    if (c->dcdf) hipFree(c->dcdf);
+    c->dfunc = c->dcdf = nullptr;
+    void* p = nullptr;
+    HIP_TRY(c, hipMalloc(&p, sizeof(float) * n));
+    c->dfunc = (float*)p;
+    HIP_TRY(c, hipMalloc(&p, sizeof(float) * n));
+    c->dcdf = (float*)p;
+    HIP_TRY(c, hipMemsetAsync(c->dfunc, 0, sizeof(float) * n, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->dcdf, 0, sizeof(float) * n, c->stream));
+    HIP_TRY(c, noor::launch_dist_build(*c, tex, tw, th, c->dfunc, c->dcdf, W, H));
+    c->S.dfunc = c->dfunc;
+    c->S.dcdf = c->dcdf;
+    c->S.dw = W;
+    c->S.dh = H;
+    return NOOR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* noor_status_string(noor_status s) {
+    if ((int)s < 0 || (int)s > 8) return "unknown";
+    return kStatus[(int)s];
+}
+
+int noor_abi_version(void) { return NOOR_ABI_VERSION; }
+
+int noor_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* noor_last_error(const noor_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : ""; }
+
+noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, const noor_camera* camera,
+                        const noor_spec* spec, const noor_options* options, noor_ctx** out) {
+    if (!P || !camera || !spec || !out) return NOOR_ERR_INVALID_ARG;
+    *out = nullptr;
+    noor_ctx* ctx = new (std::nothrow) noor_ctx();
+    if (!ctx) return NOOR_ERR_OUT_OF_MEMORY;
+    Context* c = &ctx->c;
+    auto bail = [&](noor_status s) {
+        noor_destroy(ctx);
+        return s;
+    };
+    noor_status st = validate(c, P);
+    if (st != NOOR_OK) return bail(st);
+    if (spec->skydome_type == NOOR_SKY_PHYSICAL && !hosek) return bail(NOOR_ERR_INVALID_ARG);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return bail(fail(c, NOOR_ERR_NO_DEVICE, "no HIP device"));
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(NOOR_ERR_NO_DEVICE);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(c, NOOR_ERR_HIP, "stream creation failed"));
+
+    // stack depth of the push-both traversal for this BVH
+    std::vector<int32_t> memo(P->num_bvh_nodes, -1);
+    bool bad = false;
+    c->stack_need = std::max(1u, stack_need(P, P->bvh_root_node, memo, 0, &bad));
+    if (bad) return bail(fail(c, NOOR_ERR_INVALID_PAYLOAD, "BVH references out of range or cyclic"));
+    if (c->stack_need > 96) return bail(fail(c, NOOR_ERR_STACK_DEPTH, "BVH needs " + std::to_string(c->stack_need) +
+                                                                      " stack entries, limit 96"));
+    noor::DevScene& S = c->S;
+    S.stack_depth = (int)c->stack_need;
+    S.root = P->bvh_root_node;
+    S.num_tris = P->num_triangles;
+
+    // triangles and BVH
+    const uint32_t nt = P->num_triangles;
+    if ((st = dev_upload(c, (noor_uint4**)&S.bvh_a, P->bvh_min_right_start, P->num_bvh_nodes)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_uint4**)&S.bvh_b, P->bvh_max_axis_count, P->num_bvh_nodes)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_uint4**)&S.wald0, P->wald_ax_nu_nv_nd, nt)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_uint4**)&S.wald1, P->wald_bnu_bnv_au_av, nt)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_uint4**)&S.wald2, P->wald_cnu_cnv_deg_mat, nt)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_uint4**)&S.attr, P->attribute_indices, nt)) != NOOR_OK) return bail(st);
+    std::vector<uint32_t> ids(nt ? nt : 1);
+    for (uint32_t i = 0; i < nt; ++i) ids[i] = P->prim_ids ? P->prim_ids[i] : i;
+    if ((st = dev_upload(c, (uint32_t**)&S.prim_ids, ids.data(), ids.size())) != NOOR_OK) return bail(st);
+    {
+        // watertight records: the three vertices of each triangle in leaf order
+        std::vector<noor_float4> tri((size_t)3 * (nt ? nt : 1));
+        for (uint32_t i = 0; i < nt; ++i) {
+            const noor_uint4 a = P->attribute_indices[i];
+            tri[3 * (size_t)i + 0] = P->vertices[a.x];
+            tri[3 * (size_t)i + 1] = P->vertices[a.y];
+            tri[3 * (size_t)i + 2] = P->vertices[a.z];
+        }
+        if ((st = dev_upload(c, (noor_float4**)&S.tri, tri.data(), tri.size())) != NOOR_OK) return bail(st);
+    }
+    if ((st = dev_upload(c, (noor_float4**)&S.vert, P->vertices, P->num_vertices)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_float4**)&S.nrm, P->normals, P->num_vertices)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_float2**)&S.uv, P->uvs, P->num_vertices)) != NOOR_OK) return bail(st);
+    {
+        std::vector<noor_float4> xf((size_t)9 * (P->num_instances ? P->num_instances : 1));
+        for (uint32_t i = 0; i < P->num_instances; ++i)
+            for (int r = 0; r < 3; ++r) {
+                xf[9 * (size_t)i + r] = P->world_to_object_row[r][i];
+                xf[9 * (size_t)i + 3 + r] = P->object_to_world_row[r][i];
+                xf[9 * (size_t)i + 6 + r] = P->normal_row[r][i];
+            }
+        if ((st = dev_upload(c, (noor_float4**)&S.xf, xf.data(), xf.size())) != NOOR_OK) return bail(st);
+    }
+    {
+        std::vector<noor::DevMat> mats(P->num_materials);
+        auto rgb = [&](uint32_t t, float* o) {
+            const float* px = P->textures[t].rgba;
+            o[0] = px[0];
+            o[1] = px[1];
+            o[2] = px[2];
+        };
+        for (uint32_t m = 0; m < P->num_materials; ++m) {
+            const noor_material& mt = P->materials[m];
+            noor::DevMat& d = mats[m];
+            float tmp[3];
+            rgb(mt.diffuse_tex_idx, d.diffuse);
+            rgb(mt.specular_tex_idx, d.specular);
+            rgb(mt.transmission_tex_idx, d.transmission);
+            rgb(mt.emittance_tex_idx, tmp);
+            for (int k = 0; k < 3; ++k) d.emittance[k] = mt.emitter_scale * tmp[k];
+            rgb(mt.ior_tex_idx, d.eta);
+            rgb(mt.k_tex_idx, d.k);
+            rgb(mt.roughness_tex_idx, tmp);
+            d.rough[0] = tmp[0];
+            d.rough[1] = tmp[1];
+            d.coat_ior = mt.coat_ior;
+            d.type = mt.type;
+        }
+        if ((st = dev_upload(c, (noor::DevMat**)&S.mats, mats.data(), mats.size())) != NOOR_OK) return bail(st);
+    }
+    if ((st = dev_upload(c, (noor_area_light**)&S.area, P->area_lights, P->num_area_lights)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_point_light**)&S.point, P->point_lights, P->num_point_lights)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_spot_light**)&S.spot, P->spot_lights, P->num_spot_lights)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (noor_distant_light**)&S.distant, P->distant_lights, P->num_distant_lights)) != NOOR_OK)
+        return bail(st);
+    S.n_area = P->num_area_lights;
+    S.n_point = P->num_point_lights;
+    S.n_spot = P->num_spot_lights;
+    S.n_distant = P->num_distant_lights;
+    S.num_lights = (int)(P->num_area_lights + P->num_point_lights + P->num_spot_lights + P->num_distant_lights +
+                         P->num_infinite_lights);
+    {
+        const noor_texture_view& env = P->textures[P->num_textures - 1];
+        if ((st = dev_upload(c, (noor_float4**)&S.env, (const noor_float4*)env.rgba, (size_t)env.width * env.height)) !=
+            NOOR_OK)
+            return bail(st);
+        S.env_w = env.width;
+        S.env_h = env.height;
+    }
+    noor_hosek hk{};
+    if (hosek) hk = *hosek;
+    c->hosek_host = hk;
+    if ((st = dev_upload(c, (noor_hosek**)&c->d_hosek, &hk, 1)) != NOOR_OK) return bail(st);
+    S.hosek = c->d_hosek;
+    if ((st = dev_upload(c, (noor::DevStats**)&c->d_stats, (const noor::DevStats*)nullptr, 1)) != NOOR_OK) return bail(st);
+    if ((st = dev_upload(c, (float4**)&c->d_lookat, (const float4*)nullptr, 1)) != NOOR_OK) return bail(st);
+    if ((st = apply_options(c, options)) != NOOR_OK) return bail(st);
+    if ((st = apply_spec(c, spec)) != NOOR_OK) return bail(st);
+    if ((st = apply_camera(c, camera)) != NOOR_OK) return bail(st);
+    if ((st = build_environment(c)) != NOOR_OK) return bail(st);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(c, NOOR_ERR_HIP, "upload failed"));
+    *out = ctx;
+    return NOOR_OK;
+}
+
+noor_status noor_destroy(noor_ctx* ctx) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    noor::wave_free(*c);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->film) hipFree(c->film);
+    if (c->sky_bake) hipFree(c->sky_bake);
+    if (c->dfunc) hipFree(c->dfunc);
+    if (c->dcdf) hipFree(c->dcdf);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete ctx;
+    return NOOR_OK;
+}
+
+noor_status noor_update_camera(noor_ctx* ctx, const noor_camera* camera) {
+    if (!ctx || !camera) return NOOR_ERR_INVALID_ARG;
+    return apply_camera(&ctx->c, camera);
+}
+
+noor_status noor_update_hosek(noor_ctx* ctx, const noor_hosek* hosek) {
+    if (!ctx || !hosek) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    c->hosek_host = *hosek;
+    HIP_TRY(c, hipMemcpyAsync(c->d_hosek, hosek, sizeof(noor_hosek), hipMemcpyHostToDevice, c->stream));
+    // a new sky needs a new bake and distribution (noor/cuda/skydome.cuh:58-71)
+    if (c->S.skydome_type == NOOR_SKY_PHYSICAL) {
+        noor_status st = build_environment(c);
+        if (st != NOOR_OK) return st;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NOOR_OK;
+}
+
+noor_status noor_update_spec(noor_ctx* ctx, const noor_spec* spec) {
+    if (!ctx || !spec) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    const int old_sky = c->S.skydome_type;
+    noor_status st = apply_spec(c, spec);
+    if (st != NOOR_OK) return st;
+    if (old_sky != c->S.skydome_type) {
+        st = build_environment(c);
+        if (st != NOOR_OK) return st;
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return NOOR_OK;
+}
+
+noor_status noor_update_options(noor_ctx* ctx, const noor_options* options) {
+    if (!ctx || !options) return NOOR_ERR_INVALID_ARG;
+    return apply_options(&ctx->c, options);
+}
+
+noor_status noor_render(noor_ctx* ctx, uint32_t* frame, uint32_t spp) {
+    if (!ctx || !frame || *frame < 1 || spp == 0) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    HIP_TRY(c, noor::launch_megakernel(*c, *frame, spp));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *frame += spp;
+    return NOOR_OK;
+}
+
+noor_status noor_render_wavefront(noor_ctx* ctx, uint32_t* frame, uint32_t spp) {
+    if (!ctx || !frame || *frame < 1 || spp == 0) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    if (c->S.mis) return fail(c, NOOR_ERR_UNSUPPORTED, "the wavefront renderer does not implement MIS");
+    HIP_TRY(c, noor::wave_render(*c, *frame, spp));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *frame += spp;
+    return NOOR_OK;
+}
+
+noor_status noor_debug_skydome(noor_ctx* ctx, uint32_t* frame) {
+    if (!ctx || !frame) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    HIP_TRY(c, noor::launch_debug_skydome(*c, *frame));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    ++*frame;
+    return NOOR_OK;
+}
+
+noor_status noor_get_lookat(noor_ctx* ctx, noor_float4* lookat) {
+    if (!ctx || !lookat) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    HIP_TRY(c, hipMemcpyAsync(lookat, c->d_lookat, sizeof(noor_float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NOOR_OK;
+}
+
+noor_status noor_read_film(noor_ctx* ctx, float* rgba, size_t count_floats) {
+    if (!ctx || !rgba) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    const size_t need = (size_t)4 * c->film_w * c->film_h;
+    if (count_floats < need) return fail(c, NOOR_ERR_INVALID_ARG, "film buffer too small");
+    HIP_TRY(c, hipMemcpyAsync(rgba, c->film, need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NOOR_OK;
+}
+
+noor_status noor_film_device_ptr(noor_ctx* ctx, void** ptr) {
+    if (!ctx || !ptr) return NOOR_ERR_INVALID_ARG;
+    *ptr = ctx->c.film;
+    return NOOR_OK;
+}
+
+noor_status noor_get_stats(noor_ctx* ctx, noor_stats* out) {
+    if (!ctx || !out) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    noor::DevStats d{};
+    HIP_TRY(c, hipMemcpyAsync(&d, c->d_stats, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::memset(out, 0, sizeof(*out));
+    out->extension_rays = d.extension_rays;
+    out->shadow_rays = d.shadow_rays;
+    out->node_visits = d.node_visits;
+    out->tri_tests = d.tri_tests;
+    out->tri_second_fetch = d.tri_second_fetch;
+    out->instance_entries = d.instance_entries;
+    if (d.overflow) return fail(c, NOOR_ERR_STACK_DEPTH, "traversal stack overflow");
+    return NOOR_OK;
+}
+
+noor_status noor_reset_stats(noor_ctx* ctx) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    HIP_TRY(c, hipMemsetAsync(c->d_stats, 0, sizeof(noor::DevStats), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NOOR_OK;
+}
+
+noor_status noor_synchronize(noor_ctx* ctx) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    HIP_TRY(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+    return NOOR_OK;
+}
+
+noor_status noor_trace(noor_ctx* ctx, const noor_ray* rays, uint32_t n, int any_hit, noor_hit* hits) {
+    if (!ctx || !rays || !hits) return NOOR_ERR_INVALID_ARG;
+    if (n == 0) return NOOR_OK;
+    Context* c = &ctx->c;
+    void *dr = nullptr, *dh = nullptr;
+    HIP_TRY(c, hipMalloc(&dr, sizeof(noor_ray) * n));
+    hipError_t e = hipMalloc(&dh, sizeof(noor_hit) * n);
+    if (e != hipSuccess) {
+        hipFree(dr);
+        return fail(c, NOOR_ERR_OUT_OF_MEMORY, "hit buffer");
+    }
+    e = hipMemcpyAsync(dr, rays, sizeof(noor_ray) * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = noor::launch_trace(*c, (const noor_ray*)dr, n, any_hit, (noor_hit*)dh);
+    if (e == hipSuccess) e = hipMemcpyAsync(hits, dh, sizeof(noor_hit) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(dr);
+    hipFree(dh);
+    if (e != hipSuccess) return fail(c, NOOR_ERR_HIP, std::string("trace: ") + hipGetErrorString(e));
+    return NOOR_OK;
+}
+
+size_t noor_abi_struct_size(const char* name) {
+    if (!name) return 0;
+    const std::string n(name);
+    if (n == "noor_camera") return sizeof(noor_camera);
+    if (n == "noor_hosek") return sizeof(noor_hosek);
+    if (n == "noor_spec") return sizeof(noor_spec);
+    if (n == "noor_material") return sizeof(noor_material);
+    if (n == "noor_shape") return sizeof(noor_shape);
+    if (n == "noor_area_light") return sizeof(noor_area_light);
+    if (n == "noor_payload_view") return sizeof(noor_payload_view);
+    if (n == "noor_options") return sizeof(noor_options);
+    if (n == "noor_stats") return sizeof(noor_stats);
+    if (n == "noor_ray") return sizeof(noor_ray);
+    if (n == "noor_hit") return sizeof(noor_hit);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1603 @@
+// Device library of the gfx950 render path: scene layout in HBM, RNG,
+// two-level BVH traversal, triangle tests, BSDF lobes, lights, sky and
+// environment sampling, camera rays and the per-bounce shading step.
+//
+// The megakernel and the wavefront kernels both call shade_bounce(), so their
+// per-path arithmetic is identical; the CPU oracle restates the same
+// reference code with the same operation order, which is what the parity
+// tests compare.  Reference citations are given per function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/noor_hip.h"
+#include "dev_math.h"
+
+namespace noor {
+
+// ---------------------------------------------------------------------------------------------
+// constants (noor/cuda/path_tracer.cuh:65-75)
+#define K_PI 3.141592654f
+#define K_2PI (2.0f * K_PI)
+#define K_INV_PI (1.0f / K_PI)
+#define K_INV_2PI (1.0f / K_2PI)
+#define K_PI_OVER_2 (K_PI / 2.0f)
+#define K_PI_OVER_4 (K_PI / 4.0f)
+#define K_ONE_MINUS_EPS (1.0f - 1.1920928955078125e-07f)
+#define K_INF __builtin_huge_valf()
+
+// BxDFType (noor/cuda/path_tracer.cuh:104-117)
+enum : int { B_REFLECTION = 1, B_TRANSMISSION = 2, B_DIFFUSE = 4, B_GLOSSY = 8, B_SPECULAR = 16,
+             B_CONDUCTOR = 32, B_DIELECTRIC = 64, B_NOOP = 128, B_ALL = 255 };
+enum : int { L_LAMBERT_R = 1, L_SPEC_R_NOOP = 2, L_MICRO_R_DIEL = 4, L_MICRO_R_COND = 5,
+             L_LAMBERT_T = 6, L_FRESNEL_SPEC = 10 };
+
+constexpr uint32_t M_EMITTER_ANY = NOOR_MAT_EMITTER | NOOR_MAT_MESHLIGHT;
+constexpr uint32_t M_TRANSPARENT = NOOR_MAT_TRANSLUCENT | NOOR_MAT_GLASS | NOOR_MAT_ROUGHGLASS;
+constexpr uint32_t M_SPECULAR = NOOR_MAT_MIRROR | NOOR_MAT_GLASS | NOOR_MAT_CLEARCOAT;
+
+// ---------------------------------------------------------------------------------------------
+// vectors (CUDA-samples helper_math semantics, IEEE ops, no contraction)
+struct f3 {
+    float x, y, z;
+};
+struct f2 {
+    float x, y;
+};
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 scl3(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 sdiv3(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 div3(f3 a, f3 b) { return f3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ f3 neg3(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 nrm3(f3 a) {
+    const float inv = 1.0f / sqrtf(dot3(a, a));
+    return scl3(a, inv);
+}
+__device__ __forceinline__ f3 nrm3s(f3 a) {
+    const float l = dot3(a, a);
+    if (l == 0.0f) return a;
+    const float inv = 1.0f / sqrtf(l);
+    return scl3(a, inv);
+}
+__device__ __forceinline__ float absdot3(f3 a, f3 b) { return fabsf(dot3(a, b)); }
+__device__ __forceinline__ bool is_black(f3 c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
+__device__ __forceinline__ float rmax(float a, float b) { return a > b ? a : (b == b ? b : a); }
+__device__ __forceinline__ float rmin(float a, float b) { return a < b ? a : (b == b ? b : a); }
+__device__ __forceinline__ float clampr(float v, float a, float b) { return rmax(a, rmin(v, b)); }
+__device__ __forceinline__ float maxcomp3(f3 f) { return rmax(rmax(f.x, f.y), f.z); }
+__device__ __forceinline__ int isign(float v) { return (0.0f < v) - (v < 0.0f); }
+__device__ __forceinline__ bool is_inf(float v) { return fabsf(v) == K_INF; }
+__device__ __forceinline__ float c3(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ f3 ld3(const float* f) { return f3{f[0], f[1], f[2]}; }
+__device__ __forceinline__ f3 fromf3(noor_float3 a) { return f3{a.x, a.y, a.z}; }
+__device__ __forceinline__ f3 fromf4(float4 a) { return f3{a.x, a.y, a.z}; }
+
+// ---------------------------------------------------------------------------------------------
+// scene layout in device memory
+
+// 1x1 material textures resolved at upload (noor/cuda/materialimp.cuh:134-214).
+struct DevMat {
+    float diffuse[3], specular[3], transmission[3], emittance[3], eta[3], k[3];
+    float rough[2];
+    float coat_ior;
+    uint32_t type;
+};
+
+struct DevScene {
+    // binary BVH in the reference layout: two uint4 per node (noor/cuda/bvh.cuh:28-140)
+    const uint4* __restrict__ bvh_a;
+    const uint4* __restrict__ bvh_b;
+    // triangle records in leaf order
+    const uint4* __restrict__ wald0;
+    const uint4* __restrict__ wald1;
+    const uint4* __restrict__ wald2;
+    const float4* __restrict__ tri;      // watertight: v0, v1, v2 per triangle
+    const uint4* __restrict__ attr;
+    const uint32_t* __restrict__ prim_ids;
+    const float4* __restrict__ vert;
+    const float4* __restrict__ nrm;
+    const float2* __restrict__ uv;
+    const float4* __restrict__ xf;       // 9 rows per instance: w2o, o2w, normal
+    const DevMat* __restrict__ mats;
+    const noor_area_light* __restrict__ area;
+    const noor_point_light* __restrict__ point;
+    const noor_spot_light* __restrict__ spot;
+    const noor_distant_light* __restrict__ distant;
+    const float4* __restrict__ env;
+    const float* __restrict__ dfunc;
+    const float* __restrict__ dcdf;
+    const noor_hosek* __restrict__ hosek;
+    uint32_t root;
+    uint32_t n_area, n_point, n_spot, n_distant;
+    int32_t num_lights;
+    uint32_t env_w, env_h;
+    int32_t dw, dh;
+    int32_t sky_enabled, skydome_type, bounces, rr, mis;
+    int32_t rng_kind, tri_test;
+    uint32_t compat;
+    int32_t stack_depth;
+    float bias, world_radius;
+    uint32_t num_tris;
+    noor_camera cam;
+};
+
+// ---------------------------------------------------------------------------------------------
+// RNG: TEA-seeded LCG (noor/cuda/rng.cuh:37-76) or PCG32 (O'Neill, pcg-random.org)
+struct Rng {
+    uint64_t state;  // PCG32 state, or the LCG state in the low 32 bits
+    uint64_t inc;
+};
+
+__device__ __forceinline__ uint32_t tea16(uint32_t v0, uint32_t v1) {
+    uint32_t s0 = 0;
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+__device__ __forceinline__ uint32_t pcg32_step(uint64_t& state, uint64_t inc) {
+    const uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+}
+__device__ __forceinline__ void rng_init(Rng& r, int kind, uint32_t gid, uint32_t frame) {
+    if (kind == NOOR_RNG_PCG32) {
+        r.state = 0;
+        r.inc = ((uint64_t)frame << 1u) | 1u;
+        pcg32_step(r.state, r.inc);
+        r.state += (uint64_t)gid;
+        pcg32_step(r.state, r.inc);
+    } else {
+        r.state = tea16(gid, frame);
+        r.inc = 0;
+    }
+}
+__device__ __forceinline__ float rng_next(Rng& r, int kind) {
+    if (kind == NOOR_RNG_PCG32) {
+        const uint32_t x = pcg32_step(r.state, r.inc);
+        return (float)(x >> 8) / (float)0x01000000;
+    }
+    uint32_t s = (uint32_t)r.state;
+    s = 1664525u * s + 1013904223u;
+    r.state = s;
+    return (float)(s & 0x00FFFFFFu) / (float)0x01000000;
+}
+
+// ---------------------------------------------------------------------------------------------
+// rays (noor/cuda/ray.cuh:26-207) plus the watertight shear constants
+struct Ray {
+    f3 o, d, inv;
+    float tmax;
+    int posneg[3];
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ void ray_update(Ray& r) {
+    r.inv = f3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+    r.posneg[0] = r.d.x > 0.0f ? 0 : 1;
+    r.posneg[1] = r.d.y > 0.0f ? 0 : 1;
+    r.posneg[2] = r.d.z > 0.0f ? 0 : 1;
+    // Woop, Benthin & Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013
+    const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) {
+        kz = 1;
+        m = ay;
+    }
+    if (az > m) kz = 2;
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    if (c3(r.d, kz) < 0.0f) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    const float dz = c3(r.d, kz);
+    r.Sx = c3(r.d, kx) / dz;
+    r.Sy = c3(r.d, ky) / dz;
+    r.Sz = 1.0f / dz;
+}
+__device__ __forceinline__ Ray mkray(f3 o, f3 d, float tmax) {
+    Ray r;
+    r.o = o;
+    r.d = d;
+    r.tmax = tmax;
+    ray_update(r);
+    return r;
+}
+__device__ __forceinline__ f3 ray_at(const Ray& r, float t) { return add3(r.o, scl3(r.d, t)); }
+
+struct Xf {
+    f3 r0, r1, r2, t;
+};
+__device__ __forceinline__ Xf xf_load(const float4* rows) {
+    const float4 a = rows[0], b = rows[1], c = rows[2];
+    Xf x;
+    x.r0 = f3{a.x, a.y, a.z};
+    x.r1 = f3{b.x, b.y, b.z};
+    x.r2 = f3{c.x, c.y, c.z};
+    x.t = f3{a.w, b.w, c.w};
+    return x;
+}
+__device__ __forceinline__ f3 xf_vec(const Xf& x, f3 v) { return f3{dot3(x.r0, v), dot3(x.r1, v), dot3(x.r2, v)}; }
+__device__ __forceinline__ f3 xf_pt(const Xf& x, f3 p) { return add3(x.t, xf_vec(x, p)); }
+__device__ __forceinline__ Xf xf_from(const noor_transform& t) {
+    Xf x;
+    x.r0 = fromf3(t.row[0]);
+    x.r1 = fromf3(t.row[1]);
+    x.r2 = fromf3(t.row[2]);
+    x.t = fromf3(t.t);
+    return x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// traversal
+
+// CudaBBox::intersect (noor/cuda/bbox.cuh:62-80); a box is entered when its
+// entry distance is <= tmax so that equal-t hits can be ordered by id.
+__device__ __forceinline__ bool box_hit(const Ray& r, uint4 mn, uint4 mx) {
+    const float lx = __uint_as_float(mn.x), ly = __uint_as_float(mn.y), lz = __uint_as_float(mn.z);
+    const float hx = __uint_as_float(mx.x), hy = __uint_as_float(mx.y), hz = __uint_as_float(mx.z);
+    const float bx0 = r.posneg[0] ? hx : lx, bx1 = r.posneg[0] ? lx : hx;
+    const float by0 = r.posneg[1] ? hy : ly, by1 = r.posneg[1] ? ly : hy;
+    const float bz0 = r.posneg[2] ? hz : lz, bz1 = r.posneg[2] ? lz : hz;
+    float tMin = (bx0 - r.o.x) * r.inv.x;
+    float tMax = (bx1 - r.o.x) * r.inv.x;
+    const float tyMin = (by0 - r.o.y) * r.inv.y;
+    const float tyMax = (by1 - r.o.y) * r.inv.y;
+    if (tMin > tyMax || tyMin > tMax) return false;
+    if (tyMin > tMin) tMin = tyMin;
+    if (tyMax < tMax) tMax = tyMax;
+    const float tzMin = (bz0 - r.o.z) * r.inv.z;
+    const float tzMax = (bz1 - r.o.z) * r.inv.z;
+    if (tMin > tzMax || tzMin > tMax) return false;
+    if (tzMin > tMin) tMin = tzMin;
+    if (tzMax < tMax) tMax = tzMax;
+    return (tMin <= r.tmax) && (tMax > 0.0f);
+}
+
+struct Counters {
+    uint32_t nodes, tris, tri2, inst;
+};
+
+// Wald projection test (noor/cuda/triangle.cuh:82-141)
+__device__ __forceinline__ bool tri_wald(const DevScene& S, const Ray& r, uint32_t i, float& to, float& uo,
+                                         float& vo, Counters& c) {
+    const uint4 a = S.wald0[i];
+    const uint4 cc = S.wald2[i];
+    if (cc.z == 1u) return false;
+    const uint32_t k = a.x;
+    const uint32_t ku = (k == 0u) ? 1u : ((k == 1u) ? 2u : 0u);
+    const uint32_t kv = (k == 0u) ? 2u : ((k == 1u) ? 0u : 1u);
+    const float ro_u = c3(r.o, ku), ro_v = c3(r.o, kv), ro_k = c3(r.o, k);
+    const float rd_u = c3(r.d, ku), rd_v = c3(r.d, kv), rd_k = c3(r.d, k);
+    const float nd = __uint_as_float(a.w), nu = __uint_as_float(a.y), nv = __uint_as_float(a.z);
+    const float denom = rd_k + nu * rd_u + nv * rd_v;
+    if (denom == 0.0f) return false;
+    const float t = (nd - ro_k - nu * ro_u - nv * ro_v) / denom;
+    if (t < 0.0f || t > r.tmax) return false;
+    const uint4 b = S.wald1[i];
+    c.tri2++;
+    const float hu = ro_u + t * rd_u - __uint_as_float(b.z);
+    const float hv = ro_v + t * rd_v - __uint_as_float(b.w);
+    const float u = hv * __uint_as_float(b.x) + hu * __uint_as_float(b.y);
+    const float v = hu * __uint_as_float(cc.x) + hv * __uint_as_float(cc.y);
+    if (!(u >= 0.0f && v >= 0.0f && u + v <= 1.0f)) return false;
+    to = t;
+    uo = u;
+    vo = v;
+    return true;
+}
+
+// Watertight test (Woop, Benthin & Wald 2013) with a double-precision
+// fallback for zero edge functions.
+__device__ __forceinline__ bool tri_watertight(const DevScene& S, const Ray& r, uint32_t i, float& to,
+                                               float& uo, float& vo) {
+    const float4* T = S.tri + 3 * (size_t)i;
+    const float4 p0 = T[0], p1 = T[1], p2 = T[2];
+    const f3 A = sub3(f3{p0.x, p0.y, p0.z}, r.o);
+    const f3 B = sub3(f3{p1.x, p1.y, p1.z}, r.o);
+    const f3 C = sub3(f3{p2.x, p2.y, p2.z}, r.o);
+    const float Az0 = c3(A, r.kz), Bz0 = c3(B, r.kz), Cz0 = c3(C, r.kz);
+    const float Ax = c3(A, r.kx) - r.Sx * Az0;
+    const float Ay = c3(A, r.ky) - r.Sy * Az0;
+    const float Bx = c3(B, r.kx) - r.Sx * Bz0;
+    const float By = c3(B, r.ky) - r.Sy * Bz0;
+    const float Cx = c3(C, r.kx) - r.Sx * Cz0;
+    const float Cy = c3(C, r.ky) - r.Sy * Cz0;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * Az0, Bz = r.Sz * Bz0, Cz = r.Sz * Cz0;
+    const float Tt = U * Az + V * Bz + W * Cz;
+    const float inv = 1.0f / det;
+    const float t = Tt * inv;
+    if (!(t >= 0.0f) || t > r.tmax) return false;
+    to = t;
+    uo = V * inv;
+    vo = W * inv;
+    return true;
+}
+
+template <bool STATS>
+__device__ __forceinline__ bool tri_test(const DevScene& S, const Ray& r, uint32_t i, float& t, float& u,
+                                         float& v, Counters& c) {
+    if (STATS) c.tris++;
+    if (S.tri_test == NOOR_TRI_WALD) return tri_wald(S, r, i, t, u, v, c);
+    return tri_watertight(S, r, i, t, u, v);
+}
+
+// analytic light shapes (noor/cuda/geometry.cuh:106-211)
+__device__ __forceinline__ bool solve_quadratic(float a, float b, float c, float& x0, float& x1) {
+    if (a == 0.0f) {
+        if (b != 0.0f) {
+            x0 = x1 = -c / b;
+            return true;
+        }
+        return false;
+    }
+    const float discrim = b * b - 4.0f * a * c;
+    if (discrim < 0.0f) return false;
+    const float sqrtD = sqrtf(discrim);
+    const float temp = (b < 0.0f) ? -0.5f * (b - sqrtD) : -0.5f * (b + sqrtD);
+    x0 = temp / a;
+    x1 = c / temp;
+    if (x0 > x1) {
+        const float t = x0;
+        x0 = x1;
+        x1 = t;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool shape_intersect(const noor_shape& sh, Ray& r) {
+    const f3 n = fromf3(sh.n), ctr = fromf3(sh.center);
+    if (sh.type == NOOR_SHAPE_QUAD || sh.type == NOOR_SHAPE_DISK) {
+        const float denom = dot3(n, r.d);
+        if (denom == 0.0f) return false;
+        const float t_hit = -(dot3(r.o, n) - dot3(ctr, n)) / denom;
+        if (t_hit > r.tmax) return false;
+        const f3 inplane = sub3(ray_at(r, t_hit), ctr);
+        if (sh.type == NOOR_SHAPE_QUAD) {
+            const f3 u = fromf3(sh.u), v = fromf3(sh.v);
+            const float t0 = dot3(inplane, u) / dot3(u, u);
+            if (t0 < 0.0f || t0 > 1.0f) return false;
+            const float t1 = dot3(inplane, v) / dot3(v, v);
+            if (t1 < 0.0f || t1 > 1.0f) return false;
+        } else {
+            if (dot3(inplane, inplane) > sh.radius2) return false;
+        }
+        r.tmax = t_hit;
+        return true;
+    }
+    const f3 o = sub3(r.o, ctr);
+    const float A = dot3(r.d, r.d), B = 2.0f * dot3(o, r.d), C = dot3(o, o) - sh.radius2;
+    float nearT, farT;
+    if (!solve_quadratic(A, B, C, nearT, farT)) return false;
+    if (!(nearT <= r.tmax && farT >= 0.0f)) return false;
+    if (nearT < 0.0f) {
+        if (farT > r.tmax) return false;
+        r.tmax = farT;
+    } else {
+        r.tmax = nearT;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool light_node_intersect(const DevScene& S, uint32_t li, Ray& r) {
+    if (li < S.n_area) return shape_intersect(S.area[li].shape, r);
+    return false;
+}
+
+struct Hit {
+    float t, u, v;
+    uint32_t prim, key, inst, kind;
+};
+
+// Per-lane traversal stack in LDS, lane-interleaved: entry d of lane l at
+// base[d * stride + l], so the 64 lanes of a wave hit 64 different banks.
+struct LdsStack {
+    uint32_t* base;
+    int stride;
+    int depth;
+    __device__ __forceinline__ uint32_t& at(int d) const { return base[d * stride]; }
+};
+
+// Two-level closest hit (noor/cuda/bvh.cuh:142-191).  Ties in t are broken
+// by (prim_id, instance); light nodes win only when strictly closer.
+template <bool STATS>
+__device__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const LdsStack& st, Counters& c,
+                            bool& overflow) {
+    int sp = 0;
+    Ray lray = ray;
+    uint32_t ins_idx = 0;
+    bool hit = false;
+    h.kind = 0;
+    h.prim = 0xFFFFFFFFu;
+    h.key = 0xFFFFFFFFu;
+    h.inst = 0xFFFFFFFFu;
+    h.t = 0.0f;
+    h.u = h.v = 0.0f;
+    st.at(sp++) = S.root;
+    while (sp > 0) {
+        const uint32_t ni = st.at(--sp);
+        const uint4 mn = S.bvh_a[ni];
+        const uint4 mx = S.bvh_b[ni];
+        const uint32_t w = mx.w;
+        if (STATS) c.nodes++;
+        if (w & NOOR_MESH_NODE_MASK) lray = ray;
+        if (w & NOOR_LIGHT_NODE_MASK) {
+            if (!box_hit(ray, mn, mx)) continue;
+            const uint32_t li = w & NOOR_LIGHT_MASK;
+            Ray tr = ray;
+            if (light_node_intersect(S, li, tr) && tr.tmax < ray.tmax) {
+                ray.tmax = tr.tmax;
+                hit = true;
+                h.t = tr.tmax;
+                h.kind = 2;
+                h.inst = li;
+                h.key = 0xFFFFFFFFu;
+                h.prim = 0xFFFFFFFFu;
+                h.u = h.v = 0.0f;
+            }
+            continue;
+        }
+        if (!box_hit(lray, mn, mx)) continue;
+        if (w & NOOR_MESH_INSTANCE_MASK) {
+            ins_idx = w & NOOR_TRANS_MASK;
+            if (STATS) c.inst++;
+            const Xf T = xf_load(S.xf + 9 * (size_t)ins_idx);
+            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
+            if (sp >= st.depth) { overflow = true; break; }
+            st.at(sp++) = mn.w;
+        } else if (w & NOOR_MESH_LEAF_MASK) {
+            const uint32_t cnt = w & NOOR_COUNT_MASK;
+            if (sp + (int)cnt > st.depth) { overflow = true; break; }
+            for (uint32_t i = 1; i <= cnt; ++i) st.at(sp++) = ni + i;
+        } else if (w & NOOR_TRI_LEAF_MASK) {
+            const uint32_t start = mn.w, end = start + (w & NOOR_TRI_COUNT_MASK);
+            for (uint32_t ti = start; ti < end; ++ti) {
+                float t, u, v;
+                if (!tri_test<STATS>(S, lray, ti, t, u, v, c)) continue;
+                const uint32_t key = S.prim_ids[ti];
+                bool take = false;
+                if (t < ray.tmax) take = true;
+                else if (t == ray.tmax) {
+                    if (!hit) take = true;
+                    else if (key < h.key || (key == h.key && ins_idx < h.inst)) take = true;
+                }
+                if (take) {
+                    hit = true;
+                    ray.tmax = t;
+                    lray.tmax = t;
+                    h.t = t;
+                    h.u = u;
+                    h.v = v;
+                    h.prim = ti;
+                    h.key = key;
+                    h.inst = ins_idx;
+                    h.kind = 1;
+                }
+            }
+        } else {
+            if (sp + 2 > st.depth) { overflow = true; break; }
+            const uint32_t axis = w & NOOR_AXIS_MASK;
+            if (lray.posneg[axis]) {
+                st.at(sp++) = ni + 1;
+                st.at(sp++) = mn.w;
+            } else {
+                st.at(sp++) = mn.w;
+                st.at(sp++) = ni + 1;
+            }
+        }
+    }
+    return hit;
+}
+
+// Any hit (noor/cuda/bvh.cuh:193-234).
+template <bool STATS>
+__device__ bool bvh_any(const DevScene& S, const Ray& ray, const LdsStack& st, Counters& c, bool& overflow) {
+    int sp = 0;
+    Ray lray = ray;
+    st.at(sp++) = S.root;
+    while (sp > 0) {
+        const uint32_t ni = st.at(--sp);
+        const uint4 mn = S.bvh_a[ni];
+        const uint4 mx = S.bvh_b[ni];
+        const uint32_t w = mx.w;
+        if (STATS) c.nodes++;
+        if (w & NOOR_MESH_NODE_MASK) lray = ray;
+        if (w & NOOR_LIGHT_NODE_MASK) {
+            if (!box_hit(ray, mn, mx)) continue;
+            Ray tr = ray;
+            if (light_node_intersect(S, w & NOOR_LIGHT_MASK, tr)) return true;
+            continue;
+        }
+        if (!box_hit(lray, mn, mx)) continue;
+        if (w & NOOR_MESH_INSTANCE_MASK) {
+            const uint32_t ins = w & NOOR_TRANS_MASK;
+            if (STATS) c.inst++;
+            const Xf T = xf_load(S.xf + 9 * (size_t)ins);
+            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
+            if (sp >= st.depth) { overflow = true; return false; }
+            st.at(sp++) = mn.w;
+        } else if (w & NOOR_MESH_LEAF_MASK) {
+            const uint32_t cnt = w & NOOR_COUNT_MASK;
+            if (sp + (int)cnt > st.depth) { overflow = true; return false; }
+            for (uint32_t i = 1; i <= cnt; ++i) st.at(sp++) = ni + i;
+        } else if (w & NOOR_TRI_LEAF_MASK) {
+            const uint32_t start = mn.w, end = start + (w & NOOR_TRI_COUNT_MASK);
+            for (uint32_t ti = start; ti < end; ++ti) {
+                float t, u, v;
+                if (tri_test<STATS>(S, lray, ti, t, u, v, c)) return true;
+            }
+        } else {
+            if (sp + 2 > st.depth) { overflow = true; return false; }
+            const uint32_t axis = w & NOOR_AXIS_MASK;
+            if (lray.posneg[axis]) {
+                st.at(sp++) = ni + 1;
+                st.at(sp++) = mn.w;
+            } else {
+                st.at(sp++) = mn.w;
+                st.at(sp++) = ni + 1;
+            }
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------------------------
+// sampling utilities (noor/cuda/utils.cuh:359-470)
+__device__ __forceinline__ f2 concentric_disk(f2 u) {
+    const f2 o = f2{2.0f * u.x - 1.0f, 2.0f * u.y - 1.0f};
+    if (o.x == 0.0f && o.y == 0.0f) return f2{0.0f, 0.0f};
+    float theta, r;
+    if (fabsf(o.x) > fabsf(o.y)) {
+        r = o.x;
+        theta = K_PI_OVER_4 * (o.y / o.x);
+    } else {
+        r = o.y;
+        theta = K_PI_OVER_2 - K_PI_OVER_4 * (o.x / o.y);
+    }
+    return f2{r * nm_cosf(theta), r * nm_sinf(theta)};
+}
+__device__ __forceinline__ f3 cosine_hemisphere(f2 u) {
+    const f2 d = concentric_disk(u);
+    const float z = sqrtf(rmax(0.0f, 1.0f - d.x * d.x - d.y * d.y));
+    return f3{d.x, d.y, z};
+}
+__device__ __forceinline__ float spherical_phi(f3 v) {
+    const float p = nm_atan2f(v.z, v.x);
+    return (p < 0.0f) ? (p + K_2PI) : p;
+}
+__device__ __forceinline__ float spherical_theta(f3 v) { return nm_acosf(clampr(v.y, -1.0f, 1.0f)); }
+__device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
+__device__ __forceinline__ void coordinate_system(f3 v1, f3& v2, f3& v3o) {
+    if (fabsf(v1.x) > fabsf(v1.y))
+        v2 = sdiv3(f3{-v1.z, 0.0f, v1.x}, sqrtf(v1.x * v1.x + v1.z * v1.z));
+    else
+        v2 = sdiv3(f3{0.0f, v1.z, -v1.y}, sqrtf(v1.y * v1.y + v1.z * v1.z));
+    v3o = cross3(v1, v2);
+}
+
+// ---------------------------------------------------------------------------------------------
+// environment: 2D distribution (noor/cuda/distribution2d.cuh:165-337)
+__device__ __forceinline__ float d_cdf(const DevScene& S, int row, int col) {
+    const long idx = (long)row * (S.dw + 1) + col;
+    const long n = (long)S.dh * (S.dw + 2) + 1;
+    if (idx < 0 || idx >= n) return 0.0f;
+    return S.dcdf[idx];
+}
+__device__ __forceinline__ float d_func(const DevScene& S, int row, int col) {
+    const long idx = (long)row * (S.dw + 1) + col;
+    const long n = (long)S.dh * (S.dw + 2) + 1;
+    if (idx < 0 || idx >= n) return 0.0f;
+    return S.dfunc[idx];
+}
+__device__ __forceinline__ float d_funcint(const DevScene& S, int row) {
+    return d_func(S, row, row == S.dh ? S.dh : S.dw);
+}
+__device__ __forceinline__ int d_find_interval(const DevScene& S, int row, int n, float x) {
+    int l = 0, r = n, interval = -1;
+    while (l <= r) {
+        const int m = l + (r - l) / 2;
+        if (d_cdf(S, row, m) <= x) {
+            l = m + 1;
+            interval = m;
+        } else {
+            r = m - 1;
+        }
+    }
+    return interval < 0 ? 0 : (interval > n - 1 ? n - 1 : interval);
+}
+__device__ __forceinline__ float d_sample1d(const DevScene& S, int row, int n, float u, float& pdf, int* off) {
+    const int offset = d_find_interval(S, row, n, u);
+    if (off) *off = offset;
+    const float func = d_func(S, row, offset);
+    const float cdf = d_cdf(S, row, offset);
+    const float cdf_n = d_cdf(S, row, offset + 1);
+    const float funcInt = d_funcint(S, row);
+    float du = u - cdf;
+    if ((cdf_n - cdf) > 0.0f) du /= (cdf_n - cdf);
+    pdf = (funcInt > 0.0f) ? func / funcInt : 0.0f;
+    return ((float)offset + du) / (float)n;
+}
+__device__ __forceinline__ f2 d_sample2d(const DevScene& S, f2 u, float& pdf) {
+    const int plus = (S.compat & NOOR_COMPAT_CDF_N_PLUS_1) ? 1 : 0;
+    float py, px;
+    int v;
+    const float y = d_sample1d(S, S.dh, S.dh + plus, u.y, py, &v);
+    const float x = d_sample1d(S, v, S.dw + plus, u.x, px, nullptr);
+    pdf = px * py;
+    return f2{x, y};
+}
+__device__ __forceinline__ float d_pdf(const DevScene& S, f2 p) {
+    int iu = (int)(p.x * (float)S.dw), iv = (int)(p.y * (float)S.dh);
+    iu = iu < 0 ? 0 : (iu > S.dw - 1 ? S.dw - 1 : iu);
+    iv = iv < 0 ? 0 : (iv > S.dh - 1 ? S.dh - 1 : iv);
+    return d_func(S, iv, iu) / d_funcint(S, S.dh);
+}
+
+// tex2D, normalized coordinates, linear filter, clamp addressing
+// (noor/cuda/texture.cuh:107-113), FP32 weights.
+__device__ __forceinline__ f3 tex_bilinear(const float4* tex, int w, int h, float u, float v) {
+    if (u != u) u = 0.0f;
+    if (v != v) v = 0.0f;
+    float x = u * (float)w - 0.5f;
+    float y = v * (float)h - 0.5f;
+    x = clampr(x, -2.0f, (float)w + 1.0f);
+    y = clampr(y, -2.0f, (float)h + 1.0f);
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int ix = (int)fx0, iy = (int)fy0;
+    const int i0 = ix < 0 ? 0 : (ix > w - 1 ? w - 1 : ix);
+    const int i1 = ix + 1 < 0 ? 0 : (ix + 1 > w - 1 ? w - 1 : ix + 1);
+    const int j0 = iy < 0 ? 0 : (iy > h - 1 ? h - 1 : iy);
+    const int j1 = iy + 1 < 0 ? 0 : (iy + 1 > h - 1 ? h - 1 : iy + 1);
+    const float4 t00 = tex[(size_t)j0 * w + i0], t10 = tex[(size_t)j0 * w + i1];
+    const float4 t01 = tex[(size_t)j1 * w + i0], t11 = tex[(size_t)j1 * w + i1];
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    f3 r;
+    r.x = by * (bx * t00.x + ax * t10.x) + ay * (bx * t01.x + ax * t11.x);
+    r.y = by * (bx * t00.y + ax * t10.y) + ay * (bx * t01.y + ax * t11.y);
+    r.z = by * (bx * t00.z + ax * t10.z) + ay * (bx * t01.z + ax * t11.z);
+    return r;
+}
+
+// Hosek-Wilkie (noor/cuda/hosek.cuh:55-101)
+__device__ __forceinline__ float hosek_rad(const noor_hosek_state& st, float theta, float gamma, int ch) {
+    const float* c = st.configs[ch];
+    const float cg = nm_cosf(gamma), cg2 = cg * cg, ct = nm_cosf(theta);
+    const float expM = nm_expf(c[4] * gamma);
+    const float mieM = (1.0f + cg2) / nm_powf((1.0f + c[8] * c[8] - 2.0f * c[8] * cg), 1.5f);
+    const float zenith = sqrtf(ct);
+    return (1.0f + c[0] * nm_expf(c[1] / (ct + 0.01f))) *
+           (c[2] + c[3] * expM + c[5] * cg2 + c[6] * mieM + c[7] * zenith) * st.radiances[ch];
+}
+__device__ __forceinline__ float inv_gamma(float v) {
+    if (v <= 0.04045f) return v * 1.f / 12.92f;
+    return nm_powf((v + 0.055f) * 1.f / 1.055f, 2.4f);
+}
+__device__ __forceinline__ f3 hosek_query(const noor_hosek& hk, f3 v) {
+    if (v.y < 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    const float theta = clampr(spherical_theta(v), 0.0f, K_PI_OVER_2);
+    const float gamma = nm_acosf(clampr(dot3(v, fromf3(hk.solar_dir)), -1.0f, 1.0f));
+    const f3 rgb = f3{hosek_rad(hk.state_r, theta, gamma, 0), hosek_rad(hk.state_g, theta, gamma, 1),
+                      hosek_rad(hk.state_b, theta, gamma, 2)};
+    if (gamma < hk.solar_radius) return scl3(rgb, hk.solar_scale);
+    const f3 sc = scl3(rgb, 0.025f);
+    return f3{inv_gamma(sc.x), inv_gamma(sc.y), inv_gamma(sc.z)};
+}
+
+// CudaSkyDomeManager::evaluate (noor/cuda/skydome.cuh:91-101)
+__device__ __forceinline__ f3 sky_eval(const DevScene& S, f3 dir) {
+    const float phi = spherical_phi(dir) * K_INV_2PI;
+    const float theta = spherical_theta(dir) * K_INV_PI;
+    if (S.skydome_type == NOOR_SKY_PHYSICAL) return hosek_query(*S.hosek, dir);
+    if (S.skydome_type == NOOR_SKY_CONSTANT) return f3{0.25f, 0.25f, 0.25f};
+    return tex_bilinear(S.env, (int)S.env_w, (int)S.env_h, phi, theta);
+}
+
+// importance_sample_dir (noor/cuda/skydome.cuh:131-140)
+__device__ __forceinline__ f3 env_sample_dir(const DevScene& S, Rng& rng, float& pdf, f2* uv_out = nullptr) {
+    f2 u;
+    u.x = rng_next(rng, S.rng_kind);
+    u.y = rng_next(rng, S.rng_kind);
+    float map_pdf = 1.0f;
+    u = d_sample2d(S, u, map_pdf);
+    if (uv_out) *uv_out = u;
+    const float phi = u.x * K_2PI, theta = u.y * K_PI;
+    const float cosT = nm_cosf(theta), sinT = nm_sinf(theta);
+    const f3 dir = nrm3(f3{sinT * nm_cosf(phi), cosT, sinT * nm_sinf(phi)});
+    pdf = (sinT == 0.0f) ? 0.0f : map_pdf / (2.0f * K_PI * K_PI * sinT);
+    return dir;
+}
+
+// ---------------------------------------------------------------------------------------------
+// surface interaction (noor/cuda/intersection.cuh:95-442)
+struct Isect {
+    f3 p, wo, wi, gn, sn, ss, ts;
+    float u, v;
+    uint32_t tri, ins, mat, type;
+};
+
+__device__ __forceinline__ bool is_emitter(const Isect& I) { return (I.type & M_EMITTER_ANY) != 0; }
+__device__ __forceinline__ bool is_specular(const Isect& I) { return (I.type & M_SPECULAR) != 0; }
+__device__ __forceinline__ bool is_transparent(const Isect& I) { return (I.type & M_TRANSPARENT) != 0; }
+__device__ __forceinline__ bool is_shadowcatcher(const Isect& I) { return (I.type & NOOR_MAT_SHADOWCATCHER) != 0; }
+
+__device__ void make_isect(const DevScene& S, const Ray& ray, const Hit& h, Isect& I) {
+    I.p = ray_at(ray, ray.tmax);
+    I.wo = nrm3(scl3(ray.d, -1.0f));
+    I.ins = h.inst;
+    I.wi = f3{0.0f, 0.0f, 0.0f};
+    I.gn = I.sn = I.ss = I.ts = f3{0.0f, 0.0f, 0.0f};
+    I.u = I.v = 0.0f;
+    if (h.kind == 2) {
+        I.type = NOOR_MAT_MESHLIGHT | NOOR_MAT_EMITTER;
+        I.tri = 0;
+        I.mat = 0;
+        return;
+    }
+    I.tri = h.prim;
+    I.type = S.wald2[h.prim].w;
+    const uint4 ai = S.attr[I.tri];
+    I.mat = ai.w;
+    if (I.type & M_EMITTER_ANY) return;
+    const float2 q0 = S.uv[ai.x], q1 = S.uv[ai.y], q2 = S.uv[ai.z];
+    const f2 uv0 = f2{q0.x, 1.0f - q0.y}, uv1 = f2{q1.x, 1.0f - q1.y}, uv2 = f2{q2.x, 1.0f - q2.y};
+    const f3 p0 = fromf4(S.vert[ai.x]), p1 = fromf4(S.vert[ai.y]), p2 = fromf4(S.vert[ai.z]);
+    const f3 n0 = fromf4(S.nrm[ai.x]), n1 = fromf4(S.nrm[ai.y]), n2 = fromf4(S.nrm[ai.z]);
+    const float4* X = S.xf + 9 * (size_t)I.ins;
+    Xf T = xf_load(X + 6);
+    T.t = f3{0.0f, 0.0f, 0.0f};
+    const float bu = h.u, bv = h.v;
+    const float w = 1.0f - bu - bv;
+    const f3 ni = add3(add3(scl3(n0, w), scl3(n1, bu)), scl3(n2, bv));
+    I.sn = nrm3s(nrm3(xf_vec(T, ni)));
+    I.u = w * uv0.x + bu * uv1.x + bv * uv2.x;
+    I.v = w * uv0.y + bu * uv1.y + bv * uv2.y;
+    const Xf Sx = xf_load(X + 3);
+    const f3 dp01 = xf_vec(Sx, sub3(p1, p0));
+    const f3 dp02 = xf_vec(Sx, sub3(p2, p0));
+    I.gn = nrm3s(cross3(dp01, dp02));
+    const f2 d01 = f2{uv1.x - uv0.x, uv1.y - uv0.y};
+    const f2 d02 = f2{uv2.x - uv0.x, uv2.y - uv0.y};
+    const float det = d01.x * d02.y - d02.x * d01.y;
+    const bool degenerate = (double)fabsf(det) < 1e-8;
+    f3 dpdu, dpdv;
+    if (!degenerate) {
+        const float invdet = 1.0f / det;
+        const float a00 = d02.y * invdet, a01 = -d01.y * invdet;
+        const float a10 = -d02.x * invdet, a11 = d01.x * invdet;
+        dpdu = add3(scl3(dp01, a00), scl3(dp02, a01));
+        dpdv = add3(scl3(dp01, a10), scl3(dp02, a11));
+        const f3 cr = cross3(dpdu, dpdv);
+        if (dot3(cr, cr) == 0.0f) coordinate_system(I.gn, dpdu, dpdv);
+    } else {
+        coordinate_system(I.gn, dpdu, dpdv);
+    }
+    const f3 n = I.sn;
+    const f3 ss = nrm3s(sub3(dpdu, scl3(n, dot3(n, dpdu))));
+    const float sign = dot3(cross3(n, ss), dpdv) < 0.0f ? -1.0f : 1.0f;
+    I.ss = ss;
+    I.ts = nrm3s(scl3(cross3(n, ss), sign));
+}
+
+__device__ __forceinline__ f3 to_local(const Isect& I, f3 v) { return f3{dot3(v, I.ss), dot3(v, I.ts), dot3(v, I.sn)}; }
+__device__ __forceinline__ f3 to_world(const Isect& I, f3 v) {
+    return add3(add3(scl3(I.ss, v.x), scl3(I.ts, v.y)), scl3(I.sn, v.z));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fresnel (noor/cuda/fresnel.cuh:58-107)
+__device__ __forceinline__ float fr_dielectric(float cosI, float etaI, float etaT) {
+    cosI = clampr(cosI, -1.0f, 1.0f);
+    if (!(cosI > 0.0f)) {
+        const float t = etaI;
+        etaI = etaT;
+        etaT = t;
+        cosI = fabsf(cosI);
+    }
+    const float sinI = sqrtf(rmax(0.0f, 1.0f - cosI * cosI));
+    const float sinT = etaI / etaT * sinI;
+    if (sinT >= 1.0f) return 1.0f;
+    const float cosT = sqrtf(rmax(0.0f, 1.0f - sinT * sinT));
+    const float Rparl = ((etaT * cosI) - (etaI * cosT)) / ((etaT * cosI) + (etaI * cosT));
+    const float Rperp = ((etaI * cosI) - (etaT * cosT)) / ((etaI * cosI) + (etaT * cosT));
+    return (Rparl * Rparl + Rperp * Rperp) / 2.0f;
+}
+__device__ __forceinline__ f3 fr_conductor(float cosI, f3 etaI, f3 etaT, f3 k) {
+    cosI = clampr(cosI, -1.0f, 1.0f);
+    const f3 eta = div3(etaT, etaI), etak = div3(k, etaI);
+    const float c2 = cosI * cosI, s2 = 1.0f - c2;
+    const f3 eta2 = mul3(eta, eta), etak2 = mul3(etak, etak);
+    const f3 t0 = sub3(sub3(eta2, etak2), f3{s2, s2, s2});
+    const f3 q = add3(mul3(t0, t0), mul3(scl3(eta2, 4.0f), etak2));
+    const f3 a2b2 = f3{sqrtf(q.x), sqrtf(q.y), sqrtf(q.z)};
+    const f3 t1 = add3(a2b2, f3{c2, c2, c2});
+    const f3 hh = scl3(add3(a2b2, t0), 0.5f);
+    const f3 a = f3{sqrtf(hh.x), sqrtf(hh.y), sqrtf(hh.z)};
+    const f3 t2 = scl3(a, 2.0f * cosI);
+    const f3 Rs = div3(sub3(t1, t2), add3(t1, t2));
+    const f3 t3 = add3(scl3(a2b2, c2), f3{s2 * s2, s2 * s2, s2 * s2});
+    const f3 t4 = scl3(t2, s2);
+    const f3 Rp = div3(mul3(Rs, sub3(t3, t4)), add3(t3, t4));
+    return scl3(add3(Rp, Rs), 0.5f);
+}
+
+// ---------------------------------------------------------------------------------------------
+// GGX / Trowbridge-Reitz (noor/cuda/distribution.cuh:58-183)
+__device__ __forceinline__ float cos2t(f3 w) { return w.z * w.z; }
+__device__ __forceinline__ float sin2t(f3 w) { return rmax(0.0f, 1.0f - cos2t(w)); }
+__device__ __forceinline__ float sint(f3 w) { return sqrtf(sin2t(w)); }
+__device__ __forceinline__ float tant(f3 w) { return sint(w) / w.z; }
+__device__ __forceinline__ float tan2t(f3 w) { return sin2t(w) / cos2t(w); }
+__device__ __forceinline__ float cosphi(f3 w) {
+    const float s = sint(w);
+    return (s == 0.0f) ? 1.0f : clampr(w.x / s, -1.0f, 1.0f);
+}
+__device__ __forceinline__ float sinphi(f3 w) {
+    const float s = sint(w);
+    return (s == 0.0f) ? 0.0f : clampr(w.y / s, -1.0f, 1.0f);
+}
+__device__ __forceinline__ float cos2p(f3 w) { return cosphi(w) * cosphi(w); }
+__device__ __forceinline__ float sin2p(f3 w) { return sinphi(w) * sinphi(w); }
+__device__ __forceinline__ bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0.0f; }
+
+struct Ggx {
+    float ax, ay;
+};
+__device__ __forceinline__ Ggx ggx_make(const DevMat& m) {
+    const float rx = m.rough[0], ry = m.rough[1];
+    return Ggx{rmax(rx * rx, 1e-3f), rmax(ry * ry, 1e-3f)};
+}
+__device__ __forceinline__ float ggx_D(Ggx g, f3 wh) {
+    const float t2 = tan2t(wh);
+    if (is_inf(t2)) return 0.0f;
+    const float c4 = cos2t(wh) * cos2t(wh);
+    const float e = (cos2p(wh) / (g.ax * g.ax) + sin2p(wh) / (g.ay * g.ay)) * t2;
+    return 1.0f / (K_PI * g.ax * g.ay * c4 * (1.0f + e) * (1.0f + e));
+}
+__device__ __forceinline__ float ggx_lambda(Ggx g, f3 w) {
+    const float at = fabsf(tant(w));
+    if (is_inf(at)) return 0.0f;
+    const float alpha = sqrtf(cos2p(w) * g.ax * g.ax + sin2p(w) * g.ay * g.ay);
+    const float a2t2 = (alpha * at) * (alpha * at);
+    return (-1.0f + sqrtf(1.0f + a2t2)) / 2.0f;
+}
+__device__ __forceinline__ float ggx_G(Ggx g, f3 wo, f3 wi) { return 1.0f / (1.0f + ggx_lambda(g, wo) + ggx_lambda(g, wi)); }
+__device__ __forceinline__ float ggx_G1(Ggx g, f3 w) { return 1.0f / (1.0f + ggx_lambda(g, w)); }
+__device__ __forceinline__ float ggx_pdf(Ggx g, f3 wo, f3 wh) {
+    return ggx_D(g, wh) * ggx_G1(g, wo) * absdot3(wo, wh) / fabsf(wo.z);
+}
+__device__ __forceinline__ void ggx_sample11(float cosT, float U1, float U2, float& sx, float& sy) {
+    if (cosT > .9999f) {
+        const float r = sqrtf(U1 / (1.0f - U1));
+        const float phi = 6.28318530718f * U2;
+        sx = r * nm_cosf(phi);
+        sy = r * nm_sinf(phi);
+        return;
+    }
+    const float sinT = sqrtf(rmax(0.0f, 1.0f - cosT * cosT));
+    const float tanT = sinT / cosT;
+    const float a = 1.0f / tanT;
+    const float G1 = 2.0f / (1.0f + sqrtf(1.0f + 1.0f / (a * a)));
+    const float A = 2.0f * U1 / G1 - 1.0f;
+    float tmp = 1.0f / (A * A - 1.0f);
+    if (tmp > 1e10f) tmp = 1e10f;
+    const float B = tanT;
+    const float D = sqrtf(rmax(B * B * tmp * tmp - (A * A - B * B) * tmp, 0.0f));
+    const float sx1 = B * tmp - D, sx2 = B * tmp + D;
+    sx = (A < 0.0f || sx2 > 1.0f / tanT) ? sx1 : sx2;
+    float Sg;
+    if (U2 > 0.5f) {
+        Sg = 1.0f;
+        U2 = 2.0f * (U2 - .5f);
+    } else {
+        Sg = -1.0f;
+        U2 = 2.0f * (.5f - U2);
+    }
+    const float z = (U2 * (U2 * (U2 * 0.27385f - 0.73369f) + 0.46341f)) /
+                    (U2 * (U2 * (U2 * 0.093073f + 0.309420f) - 1.000000f) + 0.597999f);
+    sy = Sg * z * sqrtf(1.0f + sx * sx);
+}
+__device__ __forceinline__ f3 ggx_sample_wh(Ggx g, f3 wo, f2 u) {
+    const float sg = (float)isign(wo.z);
+    const f3 wi = scl3(wo, sg);
+    const f3 st = nrm3(f3{g.ax * wi.x, g.ay * wi.y, wi.z});
+    float sx, sy;
+    ggx_sample11(st.z, u.x, u.y, sx, sy);
+    const float tmp = cosphi(st) * sx - sinphi(st) * sy;
+    sy = sinphi(st) * sx + cosphi(st) * sy;
+    sx = tmp;
+    sx *= g.ax;
+    sy *= g.ay;
+    return scl3(nrm3(f3{-sx, -sy, 1.0f}), sg);
+}
+
+// ---------------------------------------------------------------------------------------------
+// lobes and BSDF (noor/cuda/bxdf.cuh, noor/cuda/bsdf.cuh)
+struct Lobe {
+    int id, type;
+};
+struct Bsdf {
+    Lobe l[2];
+    int n;
+};
+__device__ __forceinline__ bool lobe_matches(int type, int flags) { return (type & flags) == type; }
+
+// factoryBSDF (noor/cuda/bsdf.cuh:425-461)
+__device__ __forceinline__ void make_bsdf(uint32_t mtype, Bsdf& b) {
+    const uint32_t t = mtype & ~(uint32_t)(NOOR_MAT_BUMP | NOOR_MAT_ALPHA);
+    b.n = 0;
+    const Lobe lam{L_LAMBERT_R, B_REFLECTION | B_DIFFUSE | B_DIELECTRIC};
+    const Lobe lamt{L_LAMBERT_T, B_TRANSMISSION | B_DIFFUSE | B_DIELECTRIC};
+    const Lobe mdiel{L_MICRO_R_DIEL, B_DIELECTRIC | B_REFLECTION | B_GLOSSY};
+    const Lobe mcond{L_MICRO_R_COND, B_CONDUCTOR | B_REFLECTION | B_GLOSSY};
+    const Lobe fspec{L_FRESNEL_SPEC, B_REFLECTION | B_TRANSMISSION | B_SPECULAR | B_DIELECTRIC};
+    const Lobe mirror{L_SPEC_R_NOOP, B_NOOP | B_REFLECTION | B_SPECULAR};
+    switch (t) {
+        case NOOR_MAT_DIFFUSE: b.l[b.n++] = lam; break;
+        case NOOR_MAT_TRANSLUCENT: b.l[b.n++] = lam; b.l[b.n++] = lamt; break;
+        case NOOR_MAT_GLASS: b.l[b.n++] = fspec; break;
+        case NOOR_MAT_GLOSSY: b.l[b.n++] = lam; b.l[b.n++] = mdiel; break;
+        case NOOR_MAT_METAL: b.l[b.n++] = mcond; break;
+        case NOOR_MAT_MIRROR: b.l[b.n++] = mirror; break;
+        default: break;
+    }
+}
+
+__device__ __forceinline__ f3 fresnel_eval(int ltype, const DevMat& m, float cosI) {
+    if (ltype & B_DIELECTRIC) {
+        const float F = fr_dielectric(cosI, 1.0f, m.coat_ior);
+        return f3{F, F, F};
+    }
+    if (ltype & B_CONDUCTOR) return fr_conductor(cosI, f3{1.0f, 1.0f, 1.0f}, ld3(m.eta), ld3(m.k));
+    return f3{1.0f, 1.0f, 1.0f};
+}
+__device__ __forceinline__ f3 micro_eval(int ltype, const DevMat& m, f3 wo, f3 wi) {
+    if (!same_hemi(wo, wi)) return f3{0.0f, 0.0f, 0.0f};
+    const float cO = fabsf(wo.z), cI = fabsf(wi.z);
+    if (cI == 0.0f || cO == 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    f3 wh = nrm3(add3(wi, wo));
+    if (is_black(wh)) return f3{0.0f, 0.0f, 0.0f};
+    wh = scl3(wh, (float)isign(wh.z));
+    const f3 F = fresnel_eval(ltype, m, dot3(wo, wh));
+    const Ggx g = ggx_make(m);
+    const float D = ggx_D(g, wh), G = ggx_G(g, wo, wi);
+    return sdiv3(scl3(scl3(mul3(F, ld3(m.specular)), D), G), 4.0f * cI * cO);
+}
+__device__ __forceinline__ float micro_pdf(const DevMat& m, f3 wo, f3 wi) {
+    if (!same_hemi(wo, wi)) return 0.0f;
+    const f3 wh = nrm3s(add3(wo, wi));
+    const Ggx g = ggx_make(m);
+    return ggx_pdf(g, wo, wh) / (4.0f * dot3(wo, wh));
+}
+__device__ __forceinline__ f3 lobe_eval(const Lobe& L, const DevMat& m, f3 wo, f3 wi) {
+    switch (L.id) {
+        case L_LAMBERT_R:
+        case L_LAMBERT_T: return scl3(ld3(m.diffuse), K_INV_PI);
+        case L_MICRO_R_DIEL:
+        case L_MICRO_R_COND: return micro_eval(L.type, m, wo, wi);
+        default: return f3{0.0f, 0.0f, 0.0f};
+    }
+}
+__device__ __forceinline__ float lobe_pdf(const Lobe& L, const DevMat& m, f3 wo, f3 wi) {
+    switch (L.id) {
+        case L_LAMBERT_R: return same_hemi(wo, wi) ? fabsf(wi.z) * K_INV_PI : 0.0f;
+        case L_LAMBERT_T: return !same_hemi(wo, wi) ? fabsf(wi.z) * K_INV_PI : 0.0f;
+        case L_MICRO_R_DIEL:
+        case L_MICRO_R_COND: return micro_pdf(m, wo, wi);
+        default: return 0.0f;
+    }
+}
+__device__ __forceinline__ f3 refl_z(f3 wo) { return f3{-wo.x, -wo.y, wo.z}; }
+__device__ __forceinline__ bool refract_(f3 wi, f3 n, float eta, f3& wt) {
+    const float cosI = dot3(n, wi);
+    const float sin2I = rmax(0.0f, 1.0f - cosI * cosI);
+    const float sin2T = eta * eta * sin2I;
+    if (sin2T >= 1.0f) return false;
+    const float cosT = sqrtf(1.0f - sin2T);
+    wt = add3(scl3(wi, -eta), scl3(n, eta * cosI - cosT));
+    return true;
+}
+__device__ __forceinline__ f3 lobe_sample(const Lobe& L, const DevMat& m, f3 wo, f3& wi, f2 u, float& pdf,
+                                          int& sampled) {
+    switch (L.id) {
+        case L_LAMBERT_R:
+        case L_LAMBERT_T: {
+            sampled = L.type;
+            wi = cosine_hemisphere(u);
+            if (L.id == L_LAMBERT_R) {
+                if (wo.z < 0.0f) wi.z *= -1.0f;
+            } else {
+                if (wo.z > 0.0f) wi.z *= -1.0f;
+            }
+            pdf = lobe_pdf(L, m, wo, wi);
+            return lobe_eval(L, m, wo, wi);
+        }
+        case L_MICRO_R_DIEL:
+        case L_MICRO_R_COND: {
+            sampled = L.type;
+            if (wo.z == 0.0f) return f3{0.0f, 0.0f, 0.0f};
+            const Ggx g = ggx_make(m);
+            const f3 wh = ggx_sample_wh(g, wo, u);
+            wi = add3(scl3(wo, -1.0f), scl3(wh, 2.0f * dot3(wo, wh)));
+            if (!same_hemi(wo, wi)) {
+                pdf = 0.0f;
+                return f3{0.0f, 0.0f, 0.0f};
+            }
+            pdf = ggx_pdf(g, wo, wh) / (4.0f * dot3(wo, wh));
+            return micro_eval(L.type, m, wo, wi);
+        }
+        case L_SPEC_R_NOOP: {
+            sampled = L.type;
+            wi = refl_z(wo);
+            pdf = 1.0f;
+            const f3 F = fresnel_eval(L.type, m, wi.z);
+            return sdiv3(mul3(F, ld3(m.diffuse)), fabsf(wi.z));
+        }
+        case L_FRESNEL_SPEC: {
+            const float F = fr_dielectric(wo.z, 1.0f, m.coat_ior);
+            if (u.x < F) {
+                wi = refl_z(wo);
+                pdf = F;
+                sampled = B_REFLECTION | B_SPECULAR;
+                return sdiv3(scl3(ld3(m.specular), F), fabsf(wi.z));
+            }
+            const bool entering = wo.z > 0.0f;
+            const float etaT = m.coat_ior;
+            const float eta = entering ? 1.0f / etaT : etaT / 1.0f;
+            const f3 nf = (dot3(wo, f3{0.0f, 0.0f, 1.0f}) < 0.0f) ? f3{0.0f, 0.0f, -1.0f} : f3{0.0f, 0.0f, 1.0f};
+            if (!refract_(wo, nf, eta, wi)) return f3{0.0f, 0.0f, 0.0f};
+            pdf = 1.0f - F;
+            sampled = B_TRANSMISSION | B_SPECULAR;
+            return sdiv3(scl3(scl3(scl3(ld3(m.transmission), eta), eta), 1.0f - F), fabsf(wi.z));
+        }
+        default: return f3{0.0f, 0.0f, 0.0f};
+    }
+}
+
+// CudaBSDF::Eval / Pdf / Sample_f (noor/cuda/bsdf.cuh:259-356)
+__device__ f3 bsdf_eval(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
+    const f3 wo = to_local(I, woW), wi = to_local(I, wiW);
+    const bool reflect = dot3(wiW, I.gn) * dot3(woW, I.gn) > 0.0f;
+    f3 f = f3{0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < b.n; ++i)
+        if (lobe_matches(b.l[i].type, flags) &&
+            ((reflect && (b.l[i].type & B_REFLECTION)) || (!reflect && (b.l[i].type & B_TRANSMISSION))))
+            f = add3(f, lobe_eval(b.l[i], m, wo, wi));
+    return f;
+}
+__device__ float bsdf_pdf(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
+    if (b.n == 0) return 0.0f;
+    const f3 wo = to_local(I, woW), wi = to_local(I, wiW);
+    if (wo.z == 0.0f) return 0.0f;
+    float pdf = 0.0f;
+    int matching = 0;
+    for (int i = 0; i < b.n; ++i)
+        if (lobe_matches(b.l[i].type, flags)) {
+            ++matching;
+            pdf += lobe_pdf(b.l[i], m, wo, wi);
+        }
+    return matching > 0 ? pdf / (float)matching : 0.0f;
+}
+__device__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3& wiW, float& pdf,
+                          int flags, int& sampled, Rng& rng, int rk) {
+    int matching = 0;
+    for (int i = 0; i < b.n; ++i)
+        if (lobe_matches(b.l[i].type, flags)) ++matching;
+    if (matching == 0) {
+        pdf = 0.0f;
+        sampled = 0;
+        return f3{0.0f, 0.0f, 0.0f};
+    }
+    f2 u;
+    u.x = rng_next(rng, rk);
+    u.y = rng_next(rng, rk);
+    int comp = (int)floorf(u.x * (float)matching);
+    if (comp > matching - 1) comp = matching - 1;
+    const f2 ur = f2{rmin(u.x * (float)matching - (float)comp, K_ONE_MINUS_EPS), u.y};
+    int li = -1;
+    int count = comp;
+    for (int i = 0; i < b.n; ++i)
+        if (lobe_matches(b.l[i].type, flags) && count-- == 0) {
+            li = i;
+            break;
+        }
+    const Lobe L = b.l[li];
+    const f3 wo = to_local(I, woW);
+    if (wo.z == 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    pdf = 0.0f;
+    sampled = L.type;
+    f3 wi = f3{0.0f, 0.0f, 0.0f};
+    f3 f = lobe_sample(L, m, wo, wi, ur, pdf, sampled);
+    if (pdf == 0.0f) {
+        sampled = 0;
+        return f3{0.0f, 0.0f, 0.0f};
+    }
+    wiW = to_world(I, wi);
+    const bool spec = (L.type & B_SPECULAR) != 0;
+    if (!spec && matching > 1)
+        for (int i = 0; i < b.n; ++i)
+            if (i != li && lobe_matches(b.l[i].type, flags)) pdf += lobe_pdf(b.l[i], m, wo, wi);
+    if (matching > 1) pdf /= (float)matching;
+    if (!spec) {
+        const bool reflect = dot3(wiW, I.gn) * dot3(woW, I.gn) > 0.0f;
+        for (int i = 0; i < b.n; ++i)
+            if (i != li && lobe_matches(b.l[i].type, flags) &&
+                ((reflect && (b.l[i].type & B_REFLECTION)) || (!reflect && (b.l[i].type & B_TRANSMISSION))))
+                f = add3(f, lobe_eval(b.l[i], m, wo, wi));
+    }
+    return f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// lights (noor/cuda/lightimp.cuh, noor/cuda/geometry.cuh)
+struct Vis {
+    f3 from, to, wi;
+    float dist;
+};
+__device__ __forceinline__ Vis mkvis(f3 from, f3 to) {
+    Vis v;
+    v.from = from;
+    v.to = to;
+    v.wi = sub3(to, from);
+    v.dist = len3(v.wi);
+    v.wi = sdiv3(v.wi, v.dist);
+    return v;
+}
+
+// light index -> kind in setup_lights order (noor/cuda/lightimp.cuh:244-281)
+__device__ __forceinline__ int light_kind(const DevScene& S, int idx, int& local) {
+    int i = idx;
+    if (i < (int)S.n_area) { local = i; return NOOR_LIGHT_AREA; }
+    i -= (int)S.n_area;
+    if (i < (int)S.n_point) { local = i; return NOOR_LIGHT_POINT; }
+    i -= (int)S.n_point;
+    if (i < (int)S.n_spot) { local = i; return NOOR_LIGHT_SPOT; }
+    i -= (int)S.n_spot;
+    if (i < (int)S.n_distant) { local = i; return NOOR_LIGHT_DISTANT; }
+    local = 0;
+    return NOOR_LIGHT_INFINITE;
+}
+__device__ __forceinline__ bool is_delta(const DevScene& S, int idx) {
+    int l;
+    const int k = light_kind(S, idx, l);
+    return k == NOOR_LIGHT_POINT || k == NOOR_LIGHT_SPOT || k == NOOR_LIGHT_DISTANT;
+}
+__device__ __forceinline__ f3 area_Le(const noor_area_light& L, f3 w) {
+    if (L.shape.type != NOOR_SHAPE_SPHERE)
+        return (L.two_sided || dot3(w, fromf3(L.shape.n)) < 0.0f) ? fromf3(L.ke) : f3{0.0f, 0.0f, 0.0f};
+    return fromf3(L.ke);
+}
+__device__ __forceinline__ f3 light_Le(const DevScene& S, f3 w, int idx) {
+    int l;
+    switch (light_kind(S, idx, l)) {
+        case NOOR_LIGHT_AREA: return area_Le(S.area[l], w);
+        case NOOR_LIGHT_INFINITE: return S.sky_enabled ? sky_eval(S, w) : f3{0.0f, 0.0f, 0.0f};
+        default: return f3{0.0f, 0.0f, 0.0f};
+    }
+}
+
+__device__ void shape_sample(const DevScene& S, const noor_shape& sh, const Isect& I, f3& p, float& pdf,
+                             Rng& rng) {
+    const int rk = S.rng_kind;
+    const float bias = S.bias;
+    const f3 n = fromf3(sh.n);
+    if (sh.type == NOOR_SHAPE_QUAD) {
+        const float r1 = rng_next(rng, rk);
+        const float r2 = rng_next(rng, rk);
+        p = add3(add3(add3(fromf3(sh.center), scl3(fromf3(sh.u), r1)), scl3(fromf3(sh.v), r2)), scl3(n, bias));
+        const f3 wi = sub3(I.p, p);
+        const float dist2 = dot3(wi, wi);
+        if (S.compat & NOOR_COMPAT_QUAD_PDF)
+            pdf = dist2 / (absdot3(n, scl3(wi, -1.0f)) * sh.area);
+        else
+            pdf = dist2 / (absdot3(n, scl3(nrm3(wi), -1.0f)) * sh.area);
+        if (is_inf(pdf)) pdf = 0.0f;
+    } else if (sh.type == NOOR_SHAPE_DISK) {
+        f2 uu;
+        uu.x = rng_next(rng, rk);
+        uu.y = rng_next(rng, rk);
+        const f2 pd = concentric_disk(uu);
+        const f3 q = f3{pd.x * sh.radius, pd.y * sh.radius, 0.0f};
+        f3 u, v;
+        coordinate_system(n, u, v);
+        p = add3(add3(add3(add3(scl3(u, q.x), scl3(v, q.y)), scl3(n, q.z)), fromf3(sh.center)), scl3(n, bias));
+        const f3 wi = sub3(I.p, p);
+        const float dist2 = dot3(wi, wi);
+        if (S.compat & NOOR_COMPAT_QUAD_PDF)
+            pdf = dist2 / (absdot3(n, scl3(wi, -1.0f)) * sh.area);
+        else
+            pdf = dist2 / (absdot3(n, scl3(nrm3(wi), -1.0f)) * sh.area);
+        if (is_inf(pdf)) pdf = 0.0f;
+    } else {
+        f2 r;
+        r.x = rng_next(rng, rk);
+        r.y = rng_next(rng, rk);
+        const f3 ctr = fromf3(sh.center);
+        const f3 w = nrm3(sub3(ctr, I.p));
+        f3 u, v;
+        coordinate_system(w, u, v);
+        const f3 pc = sub3(I.p, ctr);
+        const float sinMax2 = sh.radius2 / dot3(pc, pc);
+        const float cosMax = sqrtf(rmax(0.0f, 1.0f - sinMax2));
+        const float cosT = (1.0f - r.x) + r.x * cosMax;
+        const float sinT = sqrtf(rmax(0.0f, 1.0f - cosT * cosT));
+        const float phi = r.y * K_2PI;
+        const float dc = len3(pc);
+        const float rad = sh.radius;
+        const float ds = dc * cosT - sqrtf(rmax(0.0f, sh.radius2 - dc * dc * sinT * sinT));
+        const float cosA = (dc * dc + sh.radius2 - ds * ds) / (2.0f * dc * rad);
+        const float sinA = sqrtf(rmax(0.0f, 1.0f - cosA * cosA));
+        const f3 N = add3(add3(scl3(neg3(u), sinA * nm_cosf(phi)), scl3(neg3(v), sinA * nm_sinf(phi))),
+                          scl3(neg3(w), cosA));
+        f3 pp = add3(ctr, scl3(N, rad));
+        pp = scl3(pp, rad / len3(sub3(pp, ctr)));
+        p = add3(pp, scl3(N, bias));
+        pdf = K_INV_2PI / (1.0f - cosMax);
+    }
+}
+
+// CudaLightManager::sample_Li (noor/cuda/lightimp.cuh:415-456)
+__device__ f3 light_sample_Li(const DevScene& S, const Isect& I, int& light_idx, Vis& vis, float& pdf, Rng& rng) {
+    pdf = 0.0f;
+    const int num_lights = S.sky_enabled ? S.num_lights : S.num_lights - 1;
+    if (num_lights <= 0) return f3{0.0f, 0.0f, 0.0f};
+    const int idx = (int)((float)num_lights * rng_next(rng, S.rng_kind));
+    light_idx = idx;
+    int l;
+    f3 Ld = f3{0.0f, 0.0f, 0.0f};
+    const float wr = S.world_radius;
+    switch (light_kind(S, idx, l)) {
+        case NOOR_LIGHT_AREA: {
+            const noor_area_light& L = S.area[l];
+            f3 p;
+            shape_sample(S, L.shape, I, p, pdf, rng);
+            if (pdf == 0.0f) break;
+            vis = mkvis(I.p, p);
+            Ld = area_Le(L, vis.wi);
+            break;
+        }
+        case NOOR_LIGHT_POINT: {
+            const noor_point_light& L = S.point[l];
+            pdf = 1.0f;
+            vis = mkvis(I.p, fromf3(L.position));
+            const f3 d = sub3(vis.to, vis.from);
+            const float d2 = dot3(d, d);
+            if (d2 == 0.0f) {
+                pdf = 0.0f;
+                break;
+            }
+            Ld = sdiv3(fromf3(L.ke), d2);
+            break;
+        }
+        case NOOR_LIGHT_SPOT: {
+            const noor_spot_light& L = S.spot[l];
+            pdf = 1.0f;
+            vis = mkvis(I.p, fromf3(L.position));
+            const f3 d = sub3(vis.to, vis.from);
+            const float d2 = dot3(d, d);
+            if (d2 == 0.0f) {
+                pdf = 0.0f;
+                break;
+            }
+            const float ct = dot3(fromf3(L.direction), vis.wi);
+            float fo;
+            if (ct < L.cosouter) fo = 0.0f;
+            else if (ct >= L.cosinner) fo = 1.0f;
+            else {
+                const float dl = (ct - L.cosouter) / (L.cosinner - L.cosouter);
+                fo = (dl * dl) * (dl * dl);
+            }
+            if (fo == 0.0f) {
+                pdf = 0.0f;
+                break;
+            }
+            Ld = sdiv3(scl3(fromf3(L.ke), fo), d2);
+            break;
+        }
+        case NOOR_LIGHT_DISTANT: {
+            const noor_distant_light& L = S.distant[l];
+            pdf = 1.0f;
+            vis = mkvis(I.p, add3(I.p, scl3(fromf3(L.direction), 2.0f * wr)));
+            Ld = fromf3(L.ke);
+            break;
+        }
+        default: {
+            const f3 wi = env_sample_dir(S, rng, pdf);
+            if (is_inf(pdf) || pdf != pdf) pdf = 0.0f;
+            vis = mkvis(I.p, add3(I.p, scl3(wi, 2.0f * wr)));
+            Ld = S.sky_enabled ? sky_eval(S, vis.wi) : f3{0.0f, 0.0f, 0.0f};
+            break;
+        }
+    }
+    if (is_inf(pdf)) pdf = 0.0f;
+    return scl3(Ld, (float)num_lights);
+}
+
+__device__ float shape_pdf_solid(const noor_shape& sh, const Isect& I, f3 wi) {
+    float pdf = 0.0f;
+    if (sh.type == NOOR_SHAPE_QUAD || sh.type == NOOR_SHAPE_DISK) {
+        Ray r = mkray(I.p, wi, K_INF);
+        if (!shape_intersect(sh, r)) return 0.0f;
+        const f3 p = ray_at(r, r.tmax);
+        const f3 d = sub3(p, I.p);
+        pdf = dot3(d, d) / (absdot3(fromf3(sh.n), scl3(r.d, -1.0f)) * sh.area);
+    } else {
+        const f3 pc = sub3(I.p, fromf3(sh.center));
+        const float sinMax2 = sh.radius2 / dot3(pc, pc);
+        const float cosMax = sqrtf(rmax(0.0f, 1.0f - sinMax2));
+        pdf = K_INV_2PI / (1.0f - cosMax);
+    }
+    return is_inf(pdf) ? 0.0f : pdf;
+}
+// CudaLightManager::pdf_Li (noor/cuda/lightimp.cuh:458-483)
+__device__ float light_pdf_Li(const DevScene& S, const Isect& I, f3 wi, int idx) {
+    int l;
+    float pdf = 0.0f;
+    switch (light_kind(S, idx, l)) {
+        case NOOR_LIGHT_AREA: pdf = shape_pdf_solid(S.area[l].shape, I, wi); break;
+        case NOOR_LIGHT_INFINITE: {
+            if (!S.sky_enabled) return 0.0f;
+            const float theta = spherical_theta(wi), phi = spherical_phi(wi);
+            const float sinT = nm_sinf(theta);
+            if (sinT == 0.0f) return 0.0f;
+            pdf = d_pdf(S, f2{phi * K_INV_2PI, theta * K_INV_PI}) / (2.0f * K_PI * K_PI * sinT);
+            break;
+        }
+        default: pdf = 0.0f;
+    }
+    return is_inf(pdf) ? 0.0f : pdf;
+}
+// CudaLightManager::intersect (noor/cuda/lightimp.cuh:398-408)
+__device__ __forceinline__ bool light_intersect(const DevScene& S, Ray& r, int idx) {
+    int l;
+    const int k = light_kind(S, idx, l);
+    if (k == NOOR_LIGHT_INFINITE && S.sky_enabled) {
+        r.tmax = 2.0f * S.world_radius;
+        return true;
+    }
+    if (k != NOOR_LIGHT_AREA) return false;
+    return shape_intersect(S.area[l].shape, r);
+}
+
+// spawnShadowRay(CudaVisibility) (noor/cuda/intersection.cuh:199-207)
+__device__ __forceinline__ Ray shadow_ray_vis(const DevScene& S, const Isect& I, const Vis& v) {
+    const f3 p = dot3(v.wi, I.gn) >= 0.0f ? add3(v.from, scl3(I.gn, S.bias)) : sub3(v.from, scl3(I.gn, S.bias));
+    const float dist = len3(sub3(p, v.to));
+    return mkray(p, v.wi, dist);
+}
+
+// ---------------------------------------------------------------------------------------------
+// camera rays (noor/cuda/cameraimp.cuh:58-269)
+__device__ Ray gen_ray(const noor_camera& cam, uint32_t x, uint32_t y, Rng& rng, int rk) {
+    const float rx = (float)x + rng_next(rng, rk) - 0.5f;
+    const float ry = (float)y + rng_next(rng, rk) - 0.5f;
+    const Xf c2w = xf_from(cam.camera_to_world);
+    const Xf r2c = xf_from(cam.raster_to_camera);
+    f3 o = f3{0.0f, 0.0f, 0.0f}, d;
+    if (cam.type == NOOR_CAMERA_ORTHO) {
+        o = xf_pt(r2c, f3{rx, ry, 0.0f});
+        d = f3{0.0f, 0.0f, -1.0f};
+    } else if (cam.type == NOOR_CAMERA_ENV) {
+        const float theta = K_PI * ry / (float)cam.h + K_PI;
+        const float phi = K_2PI * rx / (float)cam.w - K_PI_OVER_2;
+        const float sT = nm_sinf(theta), cT = nm_cosf(theta);
+        d = nrm3(nrm3(f3{sT * nm_cosf(phi), cT, sT * nm_sinf(phi)}));
+    } else {
+        const f3 pc = xf_pt(r2c, f3{rx, ry, 0.0f});
+        d = nrm3(pc);
+        if (cam.lens_radius > 0.0f) {
+            f2 u;
+            u.x = rng_next(rng, rk);
+            u.y = rng_next(rng, rk);
+            const f2 pl = concentric_disk(u);
+            const float lr = cam.lens_radius;
+            const float ft = -cam.focal_length / d.z;
+            const f3 pf = add3(o, scl3(d, ft));
+            o = f3{lr * pl.x, lr * pl.y, 0.0f};
+            d = nrm3(sub3(pf, o));
+        }
+    }
+    return mkray(xf_pt(c2w, o), nrm3(xf_vec(c2w, d)), K_INF);
+}
+
+// ---------------------------------------------------------------------------------------------
+// one bounce of the path tracer (noor/cuda/path_tracer.cu:36-72,
+// noor/cuda/accumulate.cuh:28-99, noor/cuda/direct.cuh:33-132,
+// noor/cuda/scatter.cuh:27-41).
+//
+// Next-event estimation is split in two halves so that the occlusion test can
+// run elsewhere (inline in the megakernel, in the shadow kernel of the
+// wavefront renderer): shade_bounce() returns up to two candidate
+// contributions with their shadow rays, and resolve_nee() adds
+// beta_prev * (visible_a + visible_b) to L exactly as `L += beta * direct`.
+struct PathState {
+    f3 L, beta;
+    Rng rng;
+    f3 o, d;
+    uint32_t bounce;
+    uint32_t specular;
+};
+
+struct Nee {
+    f3 a, b, beta;     // candidate contributions (before occlusion) and the throughput they scale
+    Ray ra, rb;        // their shadow rays
+    int na, nb;        // 1 when the candidate needs its occlusion test
+    int has;           // 1 when `L += beta * Ld` runs for this bounce
+};
+
+__device__ __forceinline__ void resolve_nee(PathState& ps, const Nee& n, bool occ_a, bool occ_b) {
+    if (!n.has) return;
+    f3 Ld = (n.na && !occ_a) ? n.a : f3{0.0f, 0.0f, 0.0f};
+    if (n.nb) Ld = add3(Ld, occ_b ? f3{0.0f, 0.0f, 0.0f} : n.b);
+    ps.L = add3(ps.L, mul3(n.beta, Ld));
+}
+
+// Returns true when the path continues with the ray stored in ps.
+__device__ bool shade_bounce(const DevScene& S, PathState& ps, const Ray& ray, bool hit, const Hit& h, Nee& nee,
+                             f3* lookat) {
+    const int rk = S.rng_kind;
+    nee.has = 0;
+    nee.na = 0;
+    nee.nb = 0;
+    if (!hit) {
+        if (S.sky_enabled && (ps.bounce == 0 || ps.specular))
+            ps.L = add3(ps.L, mul3(ps.beta, sky_eval(S, nrm3(ray.d))));
+        return false;
+    }
+    Isect I;
+    make_isect(S, ray, h, I);
+    if (ps.bounce == 0 && lookat) *lookat = I.p;
+    if (is_emitter(I) && (ps.bounce == 0 || ps.specular)) {
+        const f3 Le = (I.type & NOOR_MAT_MESHLIGHT) ? light_Le(S, ray.d, (int)I.ins) : ld3(S.mats[I.mat].emittance);
+        ps.L = add3(ps.L, mul3(ps.beta, Le));
+        return false;
+    }
+    Bsdf b;
+    make_bsdf(I.type, b);
+    const DevMat& m = S.mats[I.mat];
+    // sampleLight (noor/cuda/direct.cuh:33-67)
+    nee.has = 1;
+    nee.beta = ps.beta;
+    nee.a = f3{0.0f, 0.0f, 0.0f};
+    nee.b = f3{0.0f, 0.0f, 0.0f};
+    int light_idx = -1;
+    bool light_ok = false;
+    if (!is_specular(I)) {
+        const int flags = B_ALL & ~B_SPECULAR;
+        Vis vis;
+        float light_pdf = 0.0f;
+        const f3 Li = light_sample_Li(S, I, light_idx, vis, light_pdf, ps.rng);
+        if (!(light_pdf == 0.0f || is_black(Li))) {
+            light_ok = true;
+            f3 f = bsdf_eval(b, I, m, I.wo, vis.wi, flags);
+            if (!is_black(f)) {
+                f = scl3(f, absdot3(vis.wi, I.sn));
+                f3 c;
+                if (is_shadowcatcher(I)) c = f;
+                else if (is_delta(S, light_idx) || !S.mis) c = sdiv3(mul3(Li, f), light_pdf);
+                else {
+                    const float sp = bsdf_pdf(b, I, m, I.wo, vis.wi, flags);
+                    const float w = power_heuristic(light_pdf, sp);
+                    c = sdiv3(scl3(mul3(Li, f), w), light_pdf);
+                }
+                nee.a = c;
+                nee.ra = shadow_ray_vis(S, I, vis);
+                nee.na = 1;
+            }
+        }
+    }
+    // sampleBSDF (noor/cuda/direct.cuh:69-113), MIS only
+    if (S.mis && !is_shadowcatcher(I) && light_ok && !is_delta(S, light_idx)) {
+        const int flags = is_specular(I) ? B_ALL : (B_ALL & ~B_SPECULAR);
+        if (S.compat & NOOR_COMPAT_WASTED_DRAWS) {
+            rng_next(ps.rng, rk);
+            rng_next(ps.rng, rk);
+        }
+        float spdf = 0.0f;
+        int sampled = 0;
+        f3 wi = f3{0.0f, 0.0f, 0.0f};
+        f3 f = bsdf_sample(b, I, m, I.wo, wi, spdf, flags, sampled, ps.rng, rk);
+        if (!(is_black(f) || spdf == 0.0f)) {
+            f = scl3(f, absdot3(wi, I.sn));
+            float sw = 1.0f;
+            bool ok = true;
+            if (!(sampled & B_SPECULAR)) {
+                const float lp = light_pdf_Li(S, I, wi, light_idx);
+                if (lp == 0.0f) ok = false;
+                else sw = power_heuristic(spdf, lp);
+            }
+            if (ok) {
+                const f3 o = dot3(wi, I.gn) >= 0.0f ? add3(I.p, scl3(I.gn, S.bias)) : sub3(I.p, scl3(I.gn, S.bias));
+                Ray sr = mkray(o, wi, 2.0f * S.world_radius);
+                if (light_intersect(S, sr, light_idx)) {
+                    const Vis v2 = mkvis(I.p, ray_at(sr, sr.tmax));
+                    nee.b = sdiv3(scl3(mul3(light_Le(S, wi, light_idx), f), sw), spdf);
+                    nee.rb = shadow_ray_vis(S, I, v2);
+                    nee.nb = 1;
+                }
+            }
+        }
+    }
+    // scatter (noor/cuda/scatter.cuh:27-41)
+    if (S.compat & NOOR_COMPAT_WASTED_DRAWS) {
+        rng_next(ps.rng, rk);
+        rng_next(ps.rng, rk);
+    }
+    float pdf = 0.0f;
+    int sampled = 0;
+    f3 wi = I.wi;
+    const f3 f = bsdf_sample(b, I, m, I.wo, wi, pdf, B_ALL, sampled, ps.rng, rk);
+    I.wi = wi;
+    const f3 sc = (pdf == 0.0f) ? f3{0.0f, 0.0f, 0.0f} : sdiv3(scl3(f, absdot3(I.wi, I.sn)), pdf);
+    ps.beta = mul3(ps.beta, sc);
+    // correct_sidedness (noor/cuda/accumulate.cuh:41-45)
+    if (!is_transparent(I) && dot3(I.wi, I.gn) < 0.0f) return false;
+    // rr_terminate (noor/cuda/accumulate.cuh:28-38)
+    if ((int)ps.bounce >= S.rr) {
+        const float q = rmax(0.05f, 1.0f - maxcomp3(ps.beta));
+        if (rng_next(ps.rng, rk) < q) return false;
+        ps.beta = sdiv3(ps.beta, 1.0f - q);
+    }
+    ps.specular = is_specular(I) ? 1u : 0u;
+    // spawnRay (noor/cuda/intersection.cuh:115-189)
+    ps.o = dot3(I.wi, I.gn) >= 0.0f ? add3(I.p, scl3(I.gn, S.bias)) : sub3(I.p, scl3(I.gn, S.bias));
+    ps.d = I.wi;
+    return true;
+}
+
+// clamp (noor/cuda/path_tracer.cu:73) and the running mean of the film
+// (noor/cuda/framebuffer.cuh:66-76)
+__device__ __forceinline__ f3 final_radiance(const DevScene& S, f3 L) {
+    if (S.compat & NOOR_COMPAT_CLAMP) {
+        L.x = clampr(L.x, 0.0f, 100.f);
+        L.y = clampr(L.y, 0.0f, 100.f);
+        L.z = clampr(L.z, 0.0f, 100.f);
+    }
+    return L;
+}
+__device__ __forceinline__ float4 film_lerp(float4 old, f3 c, uint32_t frame) {
+    if (frame <= 1) old = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    const float inv = 1.0f / (float)frame;
+    float4 r;
+    r.x = old.x + inv * (c.x - old.x);
+    r.y = old.y + inv * (c.y - old.y);
+    r.z = old.z + inv * (c.z - old.z);
+    r.w = old.w + inv * (1.0f - old.w);
+    return r;
+}
+
+}  // namespace noor

@@ -1,0 +1,195 @@
+// Megakernel renderer (the drop-in for path_tracer_kernel,
+// noor/cuda/path_tracer.cu:77-100) and the hit-query test hook.
+//
+// Launch shape: 1-D blocks of 64 lanes; each wave renders an 8x8 pixel tile
+// so that primary rays of a wave are coherent.  A launch renders `spp`
+// progressive frames and keeps the pixel's film value in registers between
+// them; the per-frame running mean is applied in frame order, so the film is
+// identical to `spp` single-frame launches.  The traversal stack lives in LDS,
+// interleaved by lane.
+#include "ctx.h"
+
+namespace noor {
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext, unsigned long long shd,
+                                            const Counters& c, bool overflow) {
+    const unsigned long long e = wave_sum(ext), s = wave_sum(shd);
+    const unsigned long long nv = wave_sum(c.nodes), tt = wave_sum(c.tris), t2 = wave_sum(c.tri2),
+                             ie = wave_sum(c.inst), ov = wave_sum(overflow ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&st->extension_rays, e);
+        atomicAdd(&st->shadow_rays, s);
+        if (nv) atomicAdd(&st->node_visits, nv);
+        if (tt) atomicAdd(&st->tri_tests, tt);
+        if (t2) atomicAdd(&st->tri_second_fetch, t2);
+        if (ie) atomicAdd(&st->instance_entries, ie);
+        if (ov) atomicAdd(&st->overflow, ov);
+    }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(64) void k_megakernel(DevScene S, float4* __restrict__ film, uint32_t frame0,
+                                                   uint32_t spp, DevStats* __restrict__ stats,
+                                                   float4* __restrict__ lookat_out) {
+    extern __shared__ uint32_t smem[];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t W = S.cam.w, H = S.cam.h;
+    const uint32_t tiles_x = (W + 7) / 8;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t x = (tile % tiles_x) * 8 + (lane & 7);
+    const uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
+    const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
+    Counters cnt{0, 0, 0, 0};
+    unsigned long long ext = 0, shd = 0;
+    bool overflow = false;
+    if (x < W && y < H) {
+        const uint32_t gid = y * W + x;
+        float4 acc = film[gid];
+        f3 lookat = f3{0.0f, 0.0f, 0.0f};
+        const bool center = (gid == S.cam.center);
+        for (uint32_t s = 0; s < spp; ++s) {
+            const uint32_t frame = frame0 + s;
+            PathState ps;
+            ps.L = f3{0.0f, 0.0f, 0.0f};
+            ps.beta = f3{1.0f, 1.0f, 1.0f};
+            ps.bounce = 0;
+            ps.specular = 0;
+            rng_init(ps.rng, S.rng_kind, gid, frame);
+            Ray ray = gen_ray(S.cam, x, y, ps.rng, S.rng_kind);
+            for (int b = 0; b < S.bounces; ++b) {
+                Hit h;
+                ++ext;
+                const bool hit = bvh_closest<STATS>(S, ray, h, st, cnt, overflow);
+                Nee nee;
+                const bool cont = shade_bounce(S, ps, ray, hit, h, nee, center ? &lookat : nullptr);
+                bool oa = false, ob = false;
+                if (nee.na) {
+                    ++shd;
+                    oa = bvh_any<STATS>(S, nee.ra, st, cnt, overflow);
+                }
+                if (nee.nb) {
+                    ++shd;
+                    ob = bvh_any<STATS>(S, nee.rb, st, cnt, overflow);
+                }
+                resolve_nee(ps, nee, oa, ob);
+                if (!cont) break;
+                ray = mkray(ps.o, ps.d, K_INF);
+                ps.bounce++;
+            }
+            acc = film_lerp(acc, final_radiance(S, ps.L), frame);
+        }
+        film[gid] = acc;
+        if (center && lookat_out) *lookat_out = make_float4(lookat.x, lookat.y, lookat.z, 1.0f);
+    }
+    flush_stats(stats, ext, shd, cnt, overflow);
+}
+
+hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp) {
+    const uint32_t tiles = ((c.S.cam.w + 7) / 8) * ((c.S.cam.h + 7) / 8);
+    const size_t lds = (size_t)64 * c.S.stack_depth * sizeof(uint32_t);
+    if (c.opt.stats)
+        hipLaunchKernelGGL(k_megakernel<true>, dim3(tiles), dim3(64), lds, c.stream, c.S, c.film, frame0, spp,
+                           c.d_stats, c.d_lookat);
+    else
+        hipLaunchKernelGGL(k_megakernel<false>, dim3(tiles), dim3(64), lds, c.stream, c.S, c.film, frame0, spp,
+                           c.d_stats, c.d_lookat);
+    return hipGetLastError();
+}
+
+// noor_trace: closest or any hit for a batch of rays.
+template <bool STATS>
+__global__ __launch_bounds__(64) void k_trace(DevScene S, const noor_ray* __restrict__ rays, uint32_t n,
+                                              int any_hit, noor_hit* __restrict__ hits, DevStats* stats) {
+    extern __shared__ uint32_t smem[];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
+    Counters cnt{0, 0, 0, 0};
+    bool overflow = false;
+    unsigned long long ext = 0, shd = 0;
+    if (i < n) {
+        const noor_ray r = rays[i];
+        Ray ray = mkray(f3{r.o.x, r.o.y, r.o.z}, f3{r.d.x, r.d.y, r.d.z}, r.tmax);
+        noor_hit out;
+        out.t = out.u = out.v = 0.0f;
+        out.prim = out.prim_id = out.inst = 0xFFFFFFFFu;
+        out.kind = 0;
+        out.pad = 0;
+        if (any_hit) {
+            ++shd;
+            out.kind = bvh_any<STATS>(S, ray, st, cnt, overflow) ? 1u : 0u;
+        } else {
+            ++ext;
+            Hit h;
+            if (bvh_closest<STATS>(S, ray, h, st, cnt, overflow)) {
+                out.t = h.t;
+                out.u = h.u;
+                out.v = h.v;
+                out.kind = h.kind;
+                out.inst = h.inst;
+                if (h.kind == 1) {
+                    out.prim = h.prim;
+                    out.prim_id = h.key;
+                }
+            }
+        }
+        hits[i] = out;
+    }
+    flush_stats(stats, ext, shd, cnt, overflow);
+}
+
+hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits) {
+    const size_t lds = (size_t)64 * c.S.stack_depth * sizeof(uint32_t);
+    const uint32_t blocks = (n + 63) / 64;
+    if (c.opt.stats)
+        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(64), lds, c.stream, c.S, d_rays, n, any_hit, d_hits,
+                           c.d_stats);
+    else
+        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(64), lds, c.stream, c.S, d_rays, n, any_hit, d_hits,
+                           c.d_stats);
+    return hipGetLastError();
+}
+
+// debug_skydome (noor/cuda/path_tracer.cu:102-122): paints the environment
+// over the film; importance-sample dots when MIS is on.
+__global__ void k_debug_skydome(DevScene S, float4* film, uint32_t frame) {
+    const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= S.cam.w || y >= S.cam.h) return;
+    const uint32_t id = y * S.cam.w + x;
+    const float u = x / (float)S.cam.w, v = y / (float)S.cam.h;
+    f3 c;
+    if (S.skydome_type == NOOR_SKY_PHYSICAL) {
+        const float phi = u * K_2PI, theta = v * K_PI;
+        const float sT = nm_sinf(theta), cT = nm_cosf(theta);
+        c = hosek_query(*S.hosek, nrm3(f3{sT * nm_cosf(phi), cT, sT * nm_sinf(phi)}));
+    } else if (S.skydome_type == NOOR_SKY_CONSTANT) {
+        c = f3{0.25f, 0.25f, 0.25f};
+    } else {
+        c = tex_bilinear(S.env, (int)S.env_w, (int)S.env_h, u, v);
+    }
+    film[id] = make_float4(c.x, c.y, c.z, 1.0f);
+    if (S.mis) {
+        Rng rng;
+        rng_init(rng, S.rng_kind, id, frame);
+        float pdf;
+        f2 q;
+        q.x = rng_next(rng, S.rng_kind);
+        q.y = rng_next(rng, S.rng_kind);
+        q = d_sample2d(S, q, pdf);
+        const uint32_t px = (uint32_t)(q.x * S.cam.w), py = (uint32_t)(q.y * S.cam.h);
+        if (px < S.cam.w && py < S.cam.h) film[py * S.cam.w + px] = make_float4(1.0f, 0.0f, 0.0f, 1.0f);
+    }
+}
+
+hipError_t launch_debug_skydome(const Context& c, uint32_t frame) {
+    hipLaunchKernelGGL(k_debug_skydome, dim3((c.S.cam.w + 15) / 16, (c.S.cam.h + 15) / 16), dim3(256), 0, c.stream,
+                       c.S, c.film, frame);
+    return hipGetLastError();
+}
+
+}  // namespace noor

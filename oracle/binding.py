@@ -1,0 +1,185 @@
+"""ctypes binding of oracle/liboracle.so (TEST INFRASTRUCTURE).
+
+Used only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+as the checker or the CPU baseline.  Never imported by noor_amd/.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from noor_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("extension_rays", "shadow_rays", "node_visits", "tri_tests",
+                                          "tri_second_fetch", "instance_entries")]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    L = C.CDLL(LIB)
+    P = C.POINTER
+    vp = C.c_void_p
+    L.oracle_scene_create.argtypes = [P(abi.PayloadView), P(abi.Hosek), P(abi.Camera), P(abi.Spec), P(abi.Options),
+                                      P(vp)]
+    L.oracle_scene_destroy.argtypes = [vp]
+    L.oracle_scene_set_camera.argtypes = [vp, P(abi.Camera)]
+    L.oracle_scene_set_spec.argtypes = [vp, P(abi.Spec)]
+    L.oracle_scene_set_options.argtypes = [vp, P(abi.Options)]
+    L.oracle_scene_set_env.argtypes = [vp, P(C.c_float)]
+    L.oracle_scene_set_material_params.argtypes = [vp, C.c_int, P(C.c_float), P(C.c_float)]
+    L.oracle_scene_dist_func.restype = P(C.c_float)
+    L.oracle_scene_dist_func.argtypes = [vp, P(C.c_int), P(C.c_int)]
+    L.oracle_scene_dist_cdf.restype = P(C.c_float)
+    L.oracle_scene_dist_cdf.argtypes = [vp]
+    for sfx, real in (("f32", C.c_float), ("f64", C.c_double)):
+        f = getattr(L, f"oracle_render_{sfx}")
+        f.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P(real),
+                      P(Counters)]
+        f = getattr(L, f"oracle_trace_{sfx}")
+        f.argtypes = [vp, P(abi.Ray), C.c_uint32, C.c_int, P(abi.Hit), P(Counters)]
+    L.oracle_trace_brute_f32.argtypes = [vp, P(abi.Ray), C.c_uint32, P(abi.Hit)]
+    L.oracle_camera_ray_f32.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(abi.Ray)]
+    L.oracle_render_detached_f64.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int, P(C.c_double)]
+    L.oracle_tea16.restype = C.c_uint32
+    L.oracle_tea16.argtypes = [C.c_uint32, C.c_uint32]
+    L.oracle_tea.restype = C.c_uint32
+    L.oracle_tea.argtypes = [C.c_uint32, C.c_uint32, P(C.c_uint32), C.c_int, P(C.c_uint32)]
+    L.oracle_pcg32_seed.argtypes = [P(C.c_uint64), P(C.c_uint64), C.c_uint64, C.c_uint64]
+    L.oracle_pcg32_next.restype = C.c_uint32
+    L.oracle_pcg32_next.argtypes = [P(C.c_uint64), C.c_uint64]
+    L.oracle_hosek_radiance.restype = C.c_float
+    L.oracle_hosek_radiance.argtypes = [P(abi.HosekState), C.c_float, C.c_float, C.c_int]
+    L.oracle_hosek_query.argtypes = [P(abi.Hosek), P(C.c_float), P(C.c_float)]
+    L.oracle_tex_bilinear.restype = C.c_float
+    L.oracle_tex_bilinear.argtypes = [P(C.c_float), C.c_int, C.c_int, C.c_float, C.c_float, C.c_int]
+    for name in ("oracle_sinf", "oracle_cosf", "oracle_acosf", "oracle_expf", "oracle_logf"):
+        f = getattr(L, name)
+        f.restype = C.c_float
+        f.argtypes = [C.c_float]
+    for name in ("oracle_atan2f", "oracle_powf"):
+        f = getattr(L, name)
+        f.restype = C.c_float
+        f.argtypes = [C.c_float, C.c_float]
+    _lib = L
+    return L
+
+
+class OracleScene:
+    """The oracle's view of a workload (payload + camera + spec + options + sky)."""
+
+    def __init__(self, payload, camera, spec, options, hosek=None):
+        self.payload = payload  # keep alive
+        self._h = C.c_void_p()
+        hk = hosek if hosek is not None else abi.Hosek()
+        self._hosek = hk
+        r = lib().oracle_scene_create(C.byref(payload.view), C.byref(hk), C.byref(camera), C.byref(spec),
+                                      C.byref(options), C.byref(self._h))
+        if r != 0:
+            raise RuntimeError("oracle_scene_create failed")
+        self.camera = camera
+
+    @classmethod
+    def from_workload(cls, wl):
+        return cls(wl.payload, wl.camera, wl.spec, wl.options, wl.hosek)
+
+    def set_camera(self, cam):
+        self.camera = cam
+        lib().oracle_scene_set_camera(self._h, C.byref(cam))
+
+    def set_spec(self, spec):
+        lib().oracle_scene_set_spec(self._h, C.byref(spec))
+
+    def set_options(self, opt):
+        lib().oracle_scene_set_options(self._h, C.byref(opt))
+
+    def set_env(self, rgba):
+        self._env = np.ascontiguousarray(rgba, np.float32)
+        if lib().oracle_scene_set_env(self._h, self._env.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+            raise RuntimeError("oracle_scene_set_env failed")
+
+    def set_material(self, m, diffuse=None, rough=None):
+        d = None if diffuse is None else (C.c_float * 3)(*map(float, diffuse))
+        r = None if rough is None else (C.c_float * 2)(*map(float, rough))
+        lib().oracle_scene_set_material_params(self._h, m, d, r)
+
+    def render(self, frame0=1, spp=1, threads=0, rect=None, fp64=False):
+        """Film (H, W, 4) after spp frames starting at frame0, plus counters."""
+        if threads <= 0:
+            threads = os.cpu_count() or 1
+        x0, y0, x1, y1 = rect if rect else (0, 0, self.camera.w, self.camera.h)
+        w, h = x1 - x0, y1 - y0
+        film = np.zeros((h, w, 4), np.float64 if fp64 else np.float32)
+        cnt = Counters()
+        if fp64:
+            f = lib().oracle_render_f64
+            ptr = film.ctypes.data_as(C.POINTER(C.c_double))
+        else:
+            f = lib().oracle_render_f32
+            ptr = film.ctypes.data_as(C.POINTER(C.c_float))
+        r = f(self._h, frame0, spp, threads, x0, y0, x1, y1, ptr, C.byref(cnt))
+        if r != 0:
+            raise RuntimeError(f"oracle render failed ({r})")
+        return film, cnt
+
+    def trace(self, rays, any_hit=False, fp64=False, brute=False):
+        n = len(rays)
+        hits = (abi.Hit * n)()
+        if brute:
+            lib().oracle_trace_brute_f32(self._h, rays, n, hits)
+            return hits, None
+        cnt = Counters()
+        f = lib().oracle_trace_f64 if fp64 else lib().oracle_trace_f32
+        if f(self._h, rays, n, 1 if any_hit else 0, hits, C.byref(cnt)) != 0:
+            raise RuntimeError("oracle trace failed")
+        return hits, cnt
+
+    def camera_ray(self, x, y, frame):
+        r = abi.Ray()
+        lib().oracle_camera_ray_f32(self._h, x, y, frame, C.byref(r))
+        return r
+
+    def dist(self):
+        w = C.c_int()
+        h = C.c_int()
+        fp = lib().oracle_scene_dist_func(self._h, C.byref(w), C.byref(h))
+        n = h.value * (w.value + 2) + 1
+        func = np.ctypeslib.as_array(fp, shape=(n,)).copy()
+        cdf = np.ctypeslib.as_array(lib().oracle_scene_dist_cdf(self._h), shape=(n,)).copy()
+        return w.value, h.value, func, cdf
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().oracle_scene_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def render_detached_f64(scene_eval, scene_frozen, frame0=1, spp=1, threads=0):
+    """FP64 render sampling with `scene_frozen`, evaluating with `scene_eval`."""
+    if threads <= 0:
+        threads = os.cpu_count() or 1
+    cam = scene_eval.camera
+    film = np.zeros((cam.h, cam.w, 4), np.float64)
+    r = lib().oracle_render_detached_f64(scene_eval._h, scene_frozen._h, frame0, spp, threads,
+                                         film.ctypes.data_as(C.POINTER(C.c_double)))
+    if r != 0:
+        raise RuntimeError("detached render failed")
+    return film
