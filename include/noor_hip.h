@@ -245,8 +245,10 @@ typedef struct noor_options {
  * which are always counted. */
 typedef struct noor_stats {
     uint64_t extension_rays, shadow_rays;
+    /* traversal events of closest-hit (extension) queries */
     uint64_t node_visits, tri_tests, tri_second_fetch, instance_entries;
-    uint64_t shading_fetches, env_lookups, env_samples, film_writes;
+    /* traversal events of any-hit (shadow) queries */
+    uint64_t shadow_node_visits, shadow_tri_tests, shadow_tri_second_fetch, shadow_instance_entries;
 } noor_stats;
 
 /* ---- hit queries (test hook; noor/cuda/bvh.cuh:142-234) ------------------ */
@@ -298,6 +300,17 @@ noor_status noor_synchronize(noor_ctx* ctx);
 /* Closest-hit (any_hit = 0) or any-hit (any_hit = 1) queries on host arrays. */
 noor_status noor_trace(noor_ctx* ctx, const noor_ray* rays, uint32_t n, int any_hit,
                        noor_hit* hits);
+
+/* Live kernel timing with HIP events on the context's stream (per kernel
+ * name: launches and total milliseconds).  Timing makes the wavefront
+ * renderer launch kernels one by one instead of replaying its graph. */
+typedef struct noor_kernel_time { char name[48]; uint64_t launches; double total_ms; } noor_kernel_time;
+noor_status noor_set_timing(noor_ctx* ctx, int enable);
+noor_status noor_get_kernel_times(noor_ctx* ctx, noor_kernel_time* out, uint32_t max, uint32_t* count);
+noor_status noor_reset_kernel_times(noor_ctx* ctx);
+/* Wavefront tunables: queue capacity in paths (0 = automatic) and lanes per
+ * block of the traversal kernels (64, 128 or 256). */
+noor_status noor_set_wavefront_params(noor_ctx* ctx, uint32_t queue_capacity, int32_t block);
 
 /* sizeof() of a named ABI struct as compiled into the library (layout checks). */
 size_t noor_abi_struct_size(const char* name);

@@ -44,6 +44,11 @@ def lib():
         "noor_reset_stats": (C.c_int, [C.c_void_p]),
         "noor_synchronize": (C.c_int, [C.c_void_p]),
         "noor_trace": (C.c_int, [C.c_void_p, P(abi.Ray), C.c_uint32, C.c_int, P(abi.Hit)]),
+        "noor_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+        "noor_get_kernel_times": (C.c_int, [C.c_void_p, P(abi.KernelTime), C.c_uint32, P(C.c_uint32)]),
+        "noor_reset_kernel_times": (C.c_int, [C.c_void_p]),
+        "noor_set_wavefront_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32]),
+        "noor_abi_struct_size": (C.c_size_t, [C.c_char_p]),
         "noor_scene_build": (C.c_int, [P(abi.SceneDesc), P(abi.BvhParams), P(C.c_void_p)]),
         "noor_payload_view_get": (C.c_int, [C.c_void_p, P(abi.PayloadView)]),
         "noor_payload_stats": (C.c_int, [C.c_void_p, P(abi.BuildStats)]),

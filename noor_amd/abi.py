@@ -165,7 +165,8 @@ class Options(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, c_u64) for n in (
         "extension_rays", "shadow_rays", "node_visits", "tri_tests", "tri_second_fetch",
-        "instance_entries", "shading_fetches", "env_lookups", "env_samples", "film_writes")]
+        "instance_entries", "shadow_node_visits", "shadow_tri_tests", "shadow_tri_second_fetch",
+        "shadow_instance_entries")]
 
 
 class Ray(C.Structure):
@@ -175,6 +176,10 @@ class Ray(C.Structure):
 class Hit(C.Structure):
     _fields_ = [("t", c_f), ("u", c_f), ("v", c_f), ("prim", c_u32), ("prim_id", c_u32),
                 ("inst", c_u32), ("kind", c_u32), ("pad", c_u32)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", c_u64), ("total_ms", C.c_double)]
 
 
 class MeshDesc(C.Structure):

@@ -160,6 +160,7 @@ int nearest_pow2(int n) {
 }
 
 noor_status apply_spec(Context* c, const noor_spec* spec) {
+    c->scene_dirty = true;
     c->spec = *spec;
     c->S.sky_enabled = (spec->lighting_type & NOOR_LIGHT_SKY_MASK) ? 1 : 0;
     c->S.skydome_type = spec->skydome_type;
@@ -173,6 +174,7 @@ noor_status apply_spec(Context* c, const noor_spec* spec) {
 }
 
 noor_status apply_options(Context* c, const noor_options* o) {
+    c->scene_dirty = true;
     noor_options def{NOOR_RNG_PCG32, NOOR_TRI_WATERTIGHT, NOOR_COMPAT_ALL, 0};
     c->opt = o ? *o : def;
     if (c->opt.rng != NOOR_RNG_PCG32 && c->opt.rng != NOOR_RNG_TEA_LCG) return fail(c, NOOR_ERR_INVALID_ARG, "rng");
@@ -185,6 +187,7 @@ noor_status apply_options(Context* c, const noor_options* o) {
 }
 
 noor_status apply_camera(Context* c, const noor_camera* cam) {
+    c->scene_dirty = true;
     if (cam->w == 0 || cam->h == 0 || cam->w > 16384 || cam->h > 16384)
         return fail(c, NOOR_ERR_INVALID_ARG, "camera size");
     if (cam->type < 0 || cam->type > 2) return fail(c, NOOR_ERR_INVALID_ARG, "camera type");
@@ -206,6 +209,7 @@ noor_status apply_camera(Context* c, const noor_camera* cam) {
 }
 
 noor_status build_environment(Context* c) {
+    c->scene_dirty = true;
     // PHYSICAL: bake sin(theta) * sky into a 2048^2 texture, then the CDF of it
     // (noor/cuda/skydome.cuh:58-71); HDR: the CDF of the environment texture.
     const float4* tex = c->S.env;
@@ -294,8 +298,15 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
 
     // triangles and BVH
     const uint32_t nt = P->num_triangles;
-    if ((st = dev_upload(c, (noor_uint4**)&S.bvh_a, P->bvh_min_right_start, P->num_bvh_nodes)) != NOOR_OK) return bail(st);
-    if ((st = dev_upload(c, (noor_uint4**)&S.bvh_b, P->bvh_max_axis_count, P->num_bvh_nodes)) != NOOR_OK) return bail(st);
+    {
+        // nodes interleaved so that one node is one 32-byte fetch
+        std::vector<noor_uint4> nodes((size_t)2 * P->num_bvh_nodes);
+        for (uint32_t i = 0; i < P->num_bvh_nodes; ++i) {
+            nodes[2 * (size_t)i] = P->bvh_min_right_start[i];
+            nodes[2 * (size_t)i + 1] = P->bvh_max_axis_count[i];
+        }
+        if ((st = dev_upload(c, (noor_uint4**)&S.bvh, nodes.data(), nodes.size())) != NOOR_OK) return bail(st);
+    }
     if ((st = dev_upload(c, (noor_uint4**)&S.wald0, P->wald_ax_nu_nv_nd, nt)) != NOOR_OK) return bail(st);
     if ((st = dev_upload(c, (noor_uint4**)&S.wald1, P->wald_bnu_bnv_au_av, nt)) != NOOR_OK) return bail(st);
     if ((st = dev_upload(c, (noor_uint4**)&S.wald2, P->wald_cnu_cnv_deg_mat, nt)) != NOOR_OK) return bail(st);
@@ -378,6 +389,7 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
     c->hosek_host = hk;
     if ((st = dev_upload(c, (noor_hosek**)&c->d_hosek, &hk, 1)) != NOOR_OK) return bail(st);
     S.hosek = c->d_hosek;
+    if ((st = dev_upload(c, (noor::DevScene**)&c->d_scene, (const noor::DevScene*)nullptr, 1)) != NOOR_OK) return bail(st);
     if ((st = dev_upload(c, (noor::DevStats**)&c->d_stats, (const noor::DevStats*)nullptr, 1)) != NOOR_OK) return bail(st);
     if ((st = dev_upload(c, (float4**)&c->d_lookat, (const float4*)nullptr, 1)) != NOOR_OK) return bail(st);
     if ((st = apply_options(c, options)) != NOOR_OK) return bail(st);
@@ -447,7 +459,45 @@ noor_status noor_render(noor_ctx* ctx, uint32_t* frame, uint32_t spp) {
     Context* c = &ctx->c;
     HIP_TRY(c, noor::launch_megakernel(*c, *frame, spp));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timer.collect();
     *frame += spp;
+    return NOOR_OK;
+}
+
+noor_status noor_set_timing(noor_ctx* ctx, int enable) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    ctx->c.timer.enabled = enable != 0;
+    return NOOR_OK;
+}
+
+noor_status noor_get_kernel_times(noor_ctx* ctx, noor_kernel_time* out, uint32_t max, uint32_t* count) {
+    if (!ctx || !count) return NOOR_ERR_INVALID_ARG;
+    uint32_t n = 0;
+    for (const auto& kv : ctx->c.timer.totals) {
+        if (out && n < max) {
+            std::memset(&out[n], 0, sizeof(out[n]));
+            std::strncpy(out[n].name, kv.first.c_str(), sizeof(out[n].name) - 1);
+            out[n].launches = kv.second.first;
+            out[n].total_ms = kv.second.second;
+        }
+        ++n;
+    }
+    *count = n;
+    return NOOR_OK;
+}
+
+noor_status noor_reset_kernel_times(noor_ctx* ctx) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    ctx->c.timer.totals.clear();
+    return NOOR_OK;
+}
+
+noor_status noor_set_wavefront_params(noor_ctx* ctx, uint32_t queue_capacity, int32_t block) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    if (block != 64 && block != 128 && block != 256) return NOOR_ERR_INVALID_ARG;
+    if (queue_capacity != 0 && queue_capacity < 64) return NOOR_ERR_INVALID_ARG;
+    ctx->c.wave_capacity = queue_capacity;
+    ctx->c.wave_block = block;
     return NOOR_OK;
 }
 
@@ -457,6 +507,7 @@ noor_status noor_render_wavefront(noor_ctx* ctx, uint32_t* frame, uint32_t spp) 
     if (c->S.mis) return fail(c, NOOR_ERR_UNSUPPORTED, "the wavefront renderer does not implement MIS");
     HIP_TRY(c, noor::wave_render(*c, *frame, spp));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timer.collect();
     *frame += spp;
     return NOOR_OK;
 }
@@ -503,10 +554,14 @@ noor_status noor_get_stats(noor_ctx* ctx, noor_stats* out) {
     std::memset(out, 0, sizeof(*out));
     out->extension_rays = d.extension_rays;
     out->shadow_rays = d.shadow_rays;
-    out->node_visits = d.node_visits;
-    out->tri_tests = d.tri_tests;
-    out->tri_second_fetch = d.tri_second_fetch;
-    out->instance_entries = d.instance_entries;
+    out->node_visits = d.ext[0];
+    out->tri_tests = d.ext[1];
+    out->tri_second_fetch = d.ext[2];
+    out->instance_entries = d.ext[3];
+    out->shadow_node_visits = d.shd[0];
+    out->shadow_tri_tests = d.shd[1];
+    out->shadow_tri_second_fetch = d.shd[2];
+    out->shadow_instance_entries = d.shd[3];
     if (d.overflow) return fail(c, NOOR_ERR_STACK_DEPTH, "traversal stack overflow");
     return NOOR_OK;
 }

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -13,11 +14,66 @@ namespace noor {
 
 // Device counters, accumulated with one atomic per wave.
 struct DevStats {
-    unsigned long long extension_rays, shadow_rays, node_visits, tri_tests, tri_second_fetch, instance_entries;
+    unsigned long long extension_rays, shadow_rays;
+    unsigned long long ext[4];   // nodes, triangle tests, second Wald fetches, instance entries
+    unsigned long long shd[4];
     unsigned long long overflow;
 };
 
 struct WaveBuffers;  // wavefront queues (kernels_wave.hip)
+
+// Live kernel timing with HIP events on the context's stream.
+struct KernelTimer {
+    struct Rec {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    bool enabled = false;
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<unsigned long long, double>> totals;  // launches, ms
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hipEventCreate(&e);
+        return e;
+    }
+    void begin(hipStream_t s, const char* name) {
+        if (!enabled) return;
+        Rec r{name, get(), get()};
+        hipEventRecord(r.a, s);
+        pending.push_back(r);
+    }
+    void end(hipStream_t s) {
+        if (!enabled || pending.empty()) return;
+        hipEventRecord(pending.back().b, s);
+    }
+    // call after the stream is synchronized
+    void collect() {
+        for (auto& r : pending) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                auto& t = totals[r.name];
+                t.first += 1;
+                t.second += ms;
+            }
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    ~KernelTimer() {
+        for (auto& r : pending) {
+            hipEventDestroy(r.a);
+            hipEventDestroy(r.b);
+        }
+        for (auto e : pool) hipEventDestroy(e);
+    }
+};
 
 struct Context {
     int device = 0;
@@ -37,10 +93,26 @@ struct Context {
     noor_hosek* d_hosek = nullptr;
     DevStats* d_stats = nullptr;
     float4* d_lookat = nullptr;
+    DevScene* d_scene = nullptr;  // device copy of S read by every kernel
+    bool scene_dirty = true;
     WaveBuffers* wave = nullptr;
+    uint32_t wave_capacity = 0;   // 0: min(pixels per layer, 2^22)
+    int wave_block = 64;          // lanes per block of the persistent traversal kernels
+    KernelTimer timer;
     uint32_t stack_need = 0;
     std::string last_error;
 };
+
+// Uploads the host copy of the scene descriptor when it changed.  Called by
+// every launch wrapper before its first launch; no kernel is in flight then.
+inline hipError_t sync_scene(const Context& cc) {
+    Context& c = const_cast<Context&>(cc);
+    if (!c.scene_dirty) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(c.d_scene, &c.S, sizeof(DevScene), hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e == hipSuccess) c.scene_dirty = false;
+    return e;
+}
 
 // kernels_mega.hip
 hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp);

@@ -94,8 +94,7 @@ struct DevMat {
 
 struct DevScene {
     // binary BVH in the reference layout: two uint4 per node (noor/cuda/bvh.cuh:28-140)
-    const uint4* __restrict__ bvh_a;
-    const uint4* __restrict__ bvh_b;
+    const uint4* __restrict__ bvh;  // node i: bvh[2i] = min|right, bvh[2i+1] = max|flags
     // triangle records in leaf order
     const uint4* __restrict__ wald0;
     const uint4* __restrict__ wald1;
@@ -182,16 +181,14 @@ __device__ __forceinline__ float rng_next(Rng& r, int kind) {
 struct Ray {
     f3 o, d, inv;
     float tmax;
-    int posneg[3];
+    uint32_t pn;  // bit a set when d[a] <= 0 (the reference posneg)
     int kx, ky, kz;
     float Sx, Sy, Sz;
 };
 
 __device__ __forceinline__ void ray_update(Ray& r) {
     r.inv = f3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
-    r.posneg[0] = r.d.x > 0.0f ? 0 : 1;
-    r.posneg[1] = r.d.y > 0.0f ? 0 : 1;
-    r.posneg[2] = r.d.z > 0.0f ? 0 : 1;
+    r.pn = (r.d.x > 0.0f ? 0u : 1u) | (r.d.y > 0.0f ? 0u : 2u) | (r.d.z > 0.0f ? 0u : 4u);
     // Woop, Benthin & Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013
     const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
     int kz = 0;
@@ -259,9 +256,9 @@ __device__ __forceinline__ Xf xf_from(const noor_transform& t) {
 __device__ __forceinline__ bool box_hit(const Ray& r, uint4 mn, uint4 mx) {
     const float lx = __uint_as_float(mn.x), ly = __uint_as_float(mn.y), lz = __uint_as_float(mn.z);
     const float hx = __uint_as_float(mx.x), hy = __uint_as_float(mx.y), hz = __uint_as_float(mx.z);
-    const float bx0 = r.posneg[0] ? hx : lx, bx1 = r.posneg[0] ? lx : hx;
-    const float by0 = r.posneg[1] ? hy : ly, by1 = r.posneg[1] ? ly : hy;
-    const float bz0 = r.posneg[2] ? hz : lz, bz1 = r.posneg[2] ? lz : hz;
+    const float bx0 = (r.pn & 1u) ? hx : lx, bx1 = (r.pn & 1u) ? lx : hx;
+    const float by0 = (r.pn & 2u) ? hy : ly, by1 = (r.pn & 2u) ? ly : hy;
+    const float bz0 = (r.pn & 4u) ? hz : lz, bz1 = (r.pn & 4u) ? lz : hz;
     float tMin = (bx0 - r.o.x) * r.inv.x;
     float tMax = (bx1 - r.o.x) * r.inv.x;
     const float tyMin = (by0 - r.o.y) * r.inv.y;
@@ -435,7 +432,7 @@ struct LdsStack {
 // Two-level closest hit (noor/cuda/bvh.cuh:142-191).  Ties in t are broken
 // by (prim_id, instance); light nodes win only when strictly closer.
 template <bool STATS>
-__device__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const LdsStack& st, Counters& c,
+__device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const LdsStack& st, Counters& c,
                             bool& overflow) {
     int sp = 0;
     Ray lray = ray;
@@ -450,8 +447,8 @@ __device__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const LdsStack&
     st.at(sp++) = S.root;
     while (sp > 0) {
         const uint32_t ni = st.at(--sp);
-        const uint4 mn = S.bvh_a[ni];
-        const uint4 mx = S.bvh_b[ni];
+        const uint4 mn = S.bvh[2 * (size_t)ni];
+        const uint4 mx = S.bvh[2 * (size_t)ni + 1];
         const uint32_t w = mx.w;
         if (STATS) c.nodes++;
         if (w & NOOR_MESH_NODE_MASK) lray = ray;
@@ -511,7 +508,7 @@ __device__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const LdsStack&
         } else {
             if (sp + 2 > st.depth) { overflow = true; break; }
             const uint32_t axis = w & NOOR_AXIS_MASK;
-            if (lray.posneg[axis]) {
+            if (((lray.pn >> axis) & 1u)) {
                 st.at(sp++) = ni + 1;
                 st.at(sp++) = mn.w;
             } else {
@@ -525,14 +522,14 @@ __device__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const LdsStack&
 
 // Any hit (noor/cuda/bvh.cuh:193-234).
 template <bool STATS>
-__device__ bool bvh_any(const DevScene& S, const Ray& ray, const LdsStack& st, Counters& c, bool& overflow) {
+__device__ __forceinline__ bool bvh_any(const DevScene& S, const Ray& ray, const LdsStack& st, Counters& c, bool& overflow) {
     int sp = 0;
     Ray lray = ray;
     st.at(sp++) = S.root;
     while (sp > 0) {
         const uint32_t ni = st.at(--sp);
-        const uint4 mn = S.bvh_a[ni];
-        const uint4 mx = S.bvh_b[ni];
+        const uint4 mn = S.bvh[2 * (size_t)ni];
+        const uint4 mx = S.bvh[2 * (size_t)ni + 1];
         const uint32_t w = mx.w;
         if (STATS) c.nodes++;
         if (w & NOOR_MESH_NODE_MASK) lray = ray;
@@ -563,7 +560,7 @@ __device__ bool bvh_any(const DevScene& S, const Ray& ray, const LdsStack& st, C
         } else {
             if (sp + 2 > st.depth) { overflow = true; return false; }
             const uint32_t axis = w & NOOR_AXIS_MASK;
-            if (lray.posneg[axis]) {
+            if (((lray.pn >> axis) & 1u)) {
                 st.at(sp++) = ni + 1;
                 st.at(sp++) = mn.w;
             } else {
@@ -755,7 +752,7 @@ __device__ __forceinline__ bool is_specular(const Isect& I) { return (I.type & M
 __device__ __forceinline__ bool is_transparent(const Isect& I) { return (I.type & M_TRANSPARENT) != 0; }
 __device__ __forceinline__ bool is_shadowcatcher(const Isect& I) { return (I.type & NOOR_MAT_SHADOWCATCHER) != 0; }
 
-__device__ void make_isect(const DevScene& S, const Ray& ray, const Hit& h, Isect& I) {
+__device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, const Hit& h, Isect& I) {
     I.p = ray_at(ray, ray.tmax);
     I.wo = nrm3(scl3(ray.d, -1.0f));
     I.ins = h.inst;
@@ -1089,7 +1086,7 @@ __device__ __forceinline__ f3 lobe_sample(const Lobe& L, const DevMat& m, f3 wo,
 }
 
 // CudaBSDF::Eval / Pdf / Sample_f (noor/cuda/bsdf.cuh:259-356)
-__device__ f3 bsdf_eval(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
+__device__ __forceinline__ f3 bsdf_eval(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
     const f3 wo = to_local(I, woW), wi = to_local(I, wiW);
     const bool reflect = dot3(wiW, I.gn) * dot3(woW, I.gn) > 0.0f;
     f3 f = f3{0.0f, 0.0f, 0.0f};
@@ -1099,7 +1096,7 @@ __device__ f3 bsdf_eval(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, 
             f = add3(f, lobe_eval(b.l[i], m, wo, wi));
     return f;
 }
-__device__ float bsdf_pdf(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
+__device__ __forceinline__ float bsdf_pdf(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
     if (b.n == 0) return 0.0f;
     const f3 wo = to_local(I, woW), wi = to_local(I, wiW);
     if (wo.z == 0.0f) return 0.0f;
@@ -1112,7 +1109,7 @@ __device__ float bsdf_pdf(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW
         }
     return matching > 0 ? pdf / (float)matching : 0.0f;
 }
-__device__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3& wiW, float& pdf,
+__device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3& wiW, float& pdf,
                           int flags, int& sampled, Rng& rng, int rk) {
     int matching = 0;
     for (int i = 0; i < b.n; ++i)
@@ -1210,7 +1207,7 @@ __device__ __forceinline__ f3 light_Le(const DevScene& S, f3 w, int idx) {
     }
 }
 
-__device__ void shape_sample(const DevScene& S, const noor_shape& sh, const Isect& I, f3& p, float& pdf,
+__device__ __forceinline__ void shape_sample(const DevScene& S, const noor_shape& sh, const Isect& I, f3& p, float& pdf,
                              Rng& rng) {
     const int rk = S.rng_kind;
     const float bias = S.bias;
@@ -1271,7 +1268,7 @@ __device__ void shape_sample(const DevScene& S, const noor_shape& sh, const Isec
 }
 
 // CudaLightManager::sample_Li (noor/cuda/lightimp.cuh:415-456)
-__device__ f3 light_sample_Li(const DevScene& S, const Isect& I, int& light_idx, Vis& vis, float& pdf, Rng& rng) {
+__device__ __forceinline__ f3 light_sample_Li(const DevScene& S, const Isect& I, int& light_idx, Vis& vis, float& pdf, Rng& rng) {
     pdf = 0.0f;
     const int num_lights = S.sky_enabled ? S.num_lights : S.num_lights - 1;
     if (num_lights <= 0) return f3{0.0f, 0.0f, 0.0f};
@@ -1347,7 +1344,7 @@ __device__ f3 light_sample_Li(const DevScene& S, const Isect& I, int& light_idx,
     return scl3(Ld, (float)num_lights);
 }
 
-__device__ float shape_pdf_solid(const noor_shape& sh, const Isect& I, f3 wi) {
+__device__ __forceinline__ float shape_pdf_solid(const noor_shape& sh, const Isect& I, f3 wi) {
     float pdf = 0.0f;
     if (sh.type == NOOR_SHAPE_QUAD || sh.type == NOOR_SHAPE_DISK) {
         Ray r = mkray(I.p, wi, K_INF);
@@ -1364,7 +1361,7 @@ __device__ float shape_pdf_solid(const noor_shape& sh, const Isect& I, f3 wi) {
     return is_inf(pdf) ? 0.0f : pdf;
 }
 // CudaLightManager::pdf_Li (noor/cuda/lightimp.cuh:458-483)
-__device__ float light_pdf_Li(const DevScene& S, const Isect& I, f3 wi, int idx) {
+__device__ __forceinline__ float light_pdf_Li(const DevScene& S, const Isect& I, f3 wi, int idx) {
     int l;
     float pdf = 0.0f;
     switch (light_kind(S, idx, l)) {
@@ -1402,7 +1399,7 @@ __device__ __forceinline__ Ray shadow_ray_vis(const DevScene& S, const Isect& I,
 
 // ---------------------------------------------------------------------------------------------
 // camera rays (noor/cuda/cameraimp.cuh:58-269)
-__device__ Ray gen_ray(const noor_camera& cam, uint32_t x, uint32_t y, Rng& rng, int rk) {
+__device__ __forceinline__ Ray gen_ray(const noor_camera& cam, uint32_t x, uint32_t y, Rng& rng, int rk) {
     const float rx = (float)x + rng_next(rng, rk) - 0.5f;
     const float ry = (float)y + rng_next(rng, rk) - 0.5f;
     const Xf c2w = xf_from(cam.camera_to_world);
@@ -1467,7 +1464,7 @@ __device__ __forceinline__ void resolve_nee(PathState& ps, const Nee& n, bool oc
 }
 
 // Returns true when the path continues with the ray stored in ps.
-__device__ bool shade_bounce(const DevScene& S, PathState& ps, const Ray& ray, bool hit, const Hit& h, Nee& nee,
+__device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, const Ray& ray, bool hit, const Hit& h, Nee& nee,
                              f3* lookat) {
     const int rk = S.rng_kind;
     nee.has = 0;

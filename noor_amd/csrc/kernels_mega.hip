@@ -16,26 +16,30 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+__device__ __forceinline__ void flush_counters(unsigned long long* dst, const Counters& c) {
+    const unsigned long long v[4] = {wave_sum(c.nodes), wave_sum(c.tris), wave_sum(c.tri2), wave_sum(c.inst)};
+    if ((threadIdx.x & 63) == 0)
+        for (int i = 0; i < 4; ++i)
+            if (v[i]) atomicAdd(&dst[i], v[i]);
+}
+
 __device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext, unsigned long long shd,
-                                            const Counters& c, bool overflow) {
-    const unsigned long long e = wave_sum(ext), s = wave_sum(shd);
-    const unsigned long long nv = wave_sum(c.nodes), tt = wave_sum(c.tris), t2 = wave_sum(c.tri2),
-                             ie = wave_sum(c.inst), ov = wave_sum(overflow ? 1ull : 0ull);
+                                            const Counters& ce, const Counters& cs, bool overflow) {
+    const unsigned long long e = wave_sum(ext), s = wave_sum(shd), ov = wave_sum(overflow ? 1ull : 0ull);
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&st->extension_rays, e);
-        atomicAdd(&st->shadow_rays, s);
-        if (nv) atomicAdd(&st->node_visits, nv);
-        if (tt) atomicAdd(&st->tri_tests, tt);
-        if (t2) atomicAdd(&st->tri_second_fetch, t2);
-        if (ie) atomicAdd(&st->instance_entries, ie);
+        if (e) atomicAdd(&st->extension_rays, e);
+        if (s) atomicAdd(&st->shadow_rays, s);
         if (ov) atomicAdd(&st->overflow, ov);
     }
+    flush_counters(st->ext, ce);
+    flush_counters(st->shd, cs);
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(64) void k_megakernel(DevScene S, float4* __restrict__ film, uint32_t frame0,
+__global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ Sp, float4* __restrict__ film, uint32_t frame0,
                                                    uint32_t spp, DevStats* __restrict__ stats,
                                                    float4* __restrict__ lookat_out) {
+    const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t W = S.cam.w, H = S.cam.h;
@@ -44,7 +48,7 @@ __global__ __launch_bounds__(64) void k_megakernel(DevScene S, float4* __restric
     const uint32_t x = (tile % tiles_x) * 8 + (lane & 7);
     const uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
     const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
-    Counters cnt{0, 0, 0, 0};
+    Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
     unsigned long long ext = 0, shd = 0;
     bool overflow = false;
     if (x < W && y < H) {
@@ -70,11 +74,11 @@ __global__ __launch_bounds__(64) void k_megakernel(DevScene S, float4* __restric
                 bool oa = false, ob = false;
                 if (nee.na) {
                     ++shd;
-                    oa = bvh_any<STATS>(S, nee.ra, st, cnt, overflow);
+                    oa = bvh_any<STATS>(S, nee.ra, st, cns, overflow);
                 }
                 if (nee.nb) {
                     ++shd;
-                    ob = bvh_any<STATS>(S, nee.rb, st, cnt, overflow);
+                    ob = bvh_any<STATS>(S, nee.rb, st, cns, overflow);
                 }
                 resolve_nee(ps, nee, oa, ob);
                 if (!cont) break;
@@ -86,29 +90,35 @@ __global__ __launch_bounds__(64) void k_megakernel(DevScene S, float4* __restric
         film[gid] = acc;
         if (center && lookat_out) *lookat_out = make_float4(lookat.x, lookat.y, lookat.z, 1.0f);
     }
-    flush_stats(stats, ext, shd, cnt, overflow);
+    flush_stats(stats, ext, shd, cnt, cns, overflow);
 }
 
 hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp) {
+    hipError_t se = sync_scene(c);
+    if (se != hipSuccess) return se;
     const uint32_t tiles = ((c.S.cam.w + 7) / 8) * ((c.S.cam.h + 7) / 8);
     const size_t lds = (size_t)64 * c.S.stack_depth * sizeof(uint32_t);
+    KernelTimer& T = const_cast<KernelTimer&>(c.timer);
+    T.begin(c.stream, "megakernel");
     if (c.opt.stats)
-        hipLaunchKernelGGL(k_megakernel<true>, dim3(tiles), dim3(64), lds, c.stream, c.S, c.film, frame0, spp,
+        hipLaunchKernelGGL(k_megakernel<true>, dim3(tiles), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
                            c.d_stats, c.d_lookat);
     else
-        hipLaunchKernelGGL(k_megakernel<false>, dim3(tiles), dim3(64), lds, c.stream, c.S, c.film, frame0, spp,
+        hipLaunchKernelGGL(k_megakernel<false>, dim3(tiles), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
                            c.d_stats, c.d_lookat);
+    T.end(c.stream);
     return hipGetLastError();
 }
 
 // noor_trace: closest or any hit for a batch of rays.
 template <bool STATS>
-__global__ __launch_bounds__(64) void k_trace(DevScene S, const noor_ray* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, const noor_ray* __restrict__ rays, uint32_t n,
                                               int any_hit, noor_hit* __restrict__ hits, DevStats* stats) {
+    const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
-    Counters cnt{0, 0, 0, 0};
+    Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
     bool overflow = false;
     unsigned long long ext = 0, shd = 0;
     if (i < n) {
@@ -121,7 +131,7 @@ __global__ __launch_bounds__(64) void k_trace(DevScene S, const noor_ray* __rest
         out.pad = 0;
         if (any_hit) {
             ++shd;
-            out.kind = bvh_any<STATS>(S, ray, st, cnt, overflow) ? 1u : 0u;
+            out.kind = bvh_any<STATS>(S, ray, st, cns, overflow) ? 1u : 0u;
         } else {
             ++ext;
             Hit h;
@@ -139,24 +149,27 @@ __global__ __launch_bounds__(64) void k_trace(DevScene S, const noor_ray* __rest
         }
         hits[i] = out;
     }
-    flush_stats(stats, ext, shd, cnt, overflow);
+    flush_stats(stats, ext, shd, cnt, cns, overflow);
 }
 
 hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits) {
+    hipError_t se = sync_scene(c);
+    if (se != hipSuccess) return se;
     const size_t lds = (size_t)64 * c.S.stack_depth * sizeof(uint32_t);
     const uint32_t blocks = (n + 63) / 64;
     if (c.opt.stats)
-        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(64), lds, c.stream, c.S, d_rays, n, any_hit, d_hits,
+        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit, d_hits,
                            c.d_stats);
     else
-        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(64), lds, c.stream, c.S, d_rays, n, any_hit, d_hits,
+        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit, d_hits,
                            c.d_stats);
     return hipGetLastError();
 }
 
 // debug_skydome (noor/cuda/path_tracer.cu:102-122): paints the environment
 // over the film; importance-sample dots when MIS is on.
-__global__ void k_debug_skydome(DevScene S, float4* film, uint32_t frame) {
+__global__ void k_debug_skydome(const DevScene* __restrict__ Sp, float4* film, uint32_t frame) {
+    const DevScene& S = *Sp;
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
     const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= S.cam.w || y >= S.cam.h) return;
@@ -187,8 +200,10 @@ __global__ void k_debug_skydome(DevScene S, float4* film, uint32_t frame) {
 }
 
 hipError_t launch_debug_skydome(const Context& c, uint32_t frame) {
+    hipError_t se = sync_scene(c);
+    if (se != hipSuccess) return se;
     hipLaunchKernelGGL(k_debug_skydome, dim3((c.S.cam.w + 15) / 16, (c.S.cam.h + 15) / 16), dim3(256), 0, c.stream,
-                       c.S, c.film, frame);
+                       c.d_scene, c.film, frame);
     return hipGetLastError();
 }
 

@@ -90,6 +90,23 @@ class Renderer:
     def synchronize(self):
         self._check(lib().noor_synchronize(self._h))
 
+    def set_timing(self, enable=True):
+        self._check(lib().noor_set_timing(self._h, 1 if enable else 0))
+
+    def kernel_times(self):
+        """{kernel name: (launches, total ms)} measured with HIP events on the context's stream."""
+        n = C.c_uint32()
+        self._check(lib().noor_get_kernel_times(self._h, None, 0, C.byref(n)))
+        arr = (abi.KernelTime * max(1, n.value))()
+        self._check(lib().noor_get_kernel_times(self._h, arr, n.value, C.byref(n)))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
+
+    def reset_kernel_times(self):
+        self._check(lib().noor_reset_kernel_times(self._h))
+
+    def set_wavefront_params(self, queue_capacity=0, block=64):
+        self._check(lib().noor_set_wavefront_params(self._h, queue_capacity, block))
+
     def trace(self, rays, any_hit=False):
         n = len(rays)
         hits = (abi.Hit * n)()

@@ -74,3 +74,51 @@ def test_megakernel_image_matches_oracle(cornell_small, rng_kind):
     err = mean_rel_err(g, c)
     print(f"mean rel err {err:.3e}, identical pixels {np.mean(np.all(g == c, axis=-1)):.4f}")
     assert err <= TOL_IMAGE
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_wavefront_matches_megakernel(cornell_small, graph, monkeypatch):
+    wl = cornell_small
+    wl.options.rng = abi.RNG_PCG32
+    wl.options.tri_test = abi.TRI_WATERTIGHT
+    if not graph:
+        monkeypatch.setenv("NOOR_NO_GRAPH", "1")
+    a = Renderer.from_workload(wl)
+    a.render(spp=3)
+    b = Renderer.from_workload(wl)
+    b.render_wavefront(spp=3)
+    fa, fb = a.film(), b.film()
+    err = mean_rel_err(fb, fa)
+    print(f"wavefront vs megakernel mean rel err {err:.3e}, identical {np.mean(np.all(fa == fb, axis=-1)):.4f}")
+    assert err <= TOL_IMAGE
+    sa, sb = a.stats(), b.stats()
+    assert sa["extension_rays"] == sb["extension_rays"]
+    assert sa["shadow_rays"] == sb["shadow_rays"]
+
+
+def test_wavefront_small_queue_chunks(cornell_small):
+    """Queue capacity below the pixel count: the frame is rendered in chunks."""
+    wl = cornell_small
+    a = Renderer.from_workload(wl)
+    a.render(spp=2)
+    b = Renderer.from_workload(wl)
+    b.set_wavefront_params(queue_capacity=1024, block=128)
+    b.render_wavefront(spp=2)
+    assert mean_rel_err(b.film(), a.film()) <= TOL_IMAGE
+
+
+@pytest.mark.parametrize("name", ["procedural", "terrain"])
+def test_large_scene_megakernel_and_wavefront_vs_oracle(name):
+    """The 1M-triangle Hosek-sky scene and the 5M-triangle env-lit terrain at
+    reduced resolution: megakernel and wavefront against the oracle."""
+    fn = getattr(scenes, name)
+    wl = fn(width=160, height=90, spp=2)
+    r = Renderer.from_workload(wl)
+    r.render(spp=2)
+    g = r.film()
+    w = Renderer.from_workload(wl)
+    w.render_wavefront(spp=2)
+    c, _ = OracleScene.from_workload(wl).render(1, 2)
+    e1, e2 = mean_rel_err(g, c), mean_rel_err(w.film(), c)
+    print(f"{name}: megakernel {e1:.3e} wavefront {e2:.3e}")
+    assert e1 <= TOL_IMAGE and e2 <= TOL_IMAGE
