@@ -117,13 +117,16 @@ class OracleScene:
         r = None if rough is None else (C.c_float * 2)(*map(float, rough))
         lib().oracle_scene_set_material_params(self._h, m, d, r)
 
-    def render(self, frame0=1, spp=1, threads=0, rect=None, fp64=False):
-        """Film (H, W, 4) after spp frames starting at frame0, plus counters."""
+    def render(self, frame0=1, spp=1, threads=0, rect=None, fp64=False, film=None):
+        """Film (H, W, 4) after spp frames starting at frame0, plus counters.
+        A film passed in is continued (its running mean is updated in place)."""
         if threads <= 0:
             threads = os.cpu_count() or 1
         x0, y0, x1, y1 = rect if rect else (0, 0, self.camera.w, self.camera.h)
         w, h = x1 - x0, y1 - y0
-        film = np.zeros((h, w, 4), np.float64 if fp64 else np.float32)
+        if film is None:
+            film = np.zeros((h, w, 4), np.float64 if fp64 else np.float32)
+        assert film.shape == (h, w, 4) and film.flags.c_contiguous
         cnt = Counters()
         if fp64:
             f = lib().oracle_render_f64
@@ -183,3 +186,80 @@ def render_detached_f64(scene_eval, scene_frozen, frame0=1, spp=1, threads=0):
     if r != 0:
         raise RuntimeError("detached render failed")
     return film
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def rng_sequence(kind, gid, frame, n):
+    out = np.zeros(n, np.float32)
+    L = lib()
+    L.oracle_rng_sequence.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
+    L.oracle_rng_sequence(kind, gid, frame, n, _fp(out))
+    return out
+
+
+def material_params(diffuse=(0.5, 0.5, 0.5), specular=(1, 1, 1), eta=(1, 1, 1), k=(0, 0, 0), rough=(0.3, 0.3),
+                    coat_ior=1.5, transmission=(1, 1, 1)):
+    return np.array([*diffuse, *specular, *eta, *k, *rough, coat_ior, *transmission], np.float32)
+
+
+def bsdf_eval(mtype, params, wo, wi):
+    L = lib()
+    L.oracle_bsdf_eval.argtypes = [C.c_uint32] + [C.POINTER(C.c_float)] * 5
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    L.oracle_bsdf_eval(mtype, _fp(params), _fp(wo), _fp(wi), _fp(f), _fp(pdf))
+    return f, float(pdf[0])
+
+
+def bsdf_sample(mtype, params, wo, gid, frame=1):
+    L = lib()
+    L.oracle_bsdf_sample.restype = C.c_int
+    L.oracle_bsdf_sample.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32,
+                                     C.c_uint32] + [C.POINTER(C.c_float)] * 3
+    wi = np.zeros(3, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    flags = L.oracle_bsdf_sample(mtype, _fp(params), _fp(wo), gid, frame, _fp(wi), _fp(f), _fp(pdf))
+    return wi, f, float(pdf[0]), flags
+
+
+def fresnel_dielectric(cos_i, eta_i, eta_t):
+    L = lib()
+    L.oracle_fresnel_dielectric.restype = C.c_float
+    L.oracle_fresnel_dielectric.argtypes = [C.c_float] * 3
+    return L.oracle_fresnel_dielectric(cos_i, eta_i, eta_t)
+
+
+def fresnel_conductor(cos_i, eta, k):
+    L = lib()
+    L.oracle_fresnel_conductor.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float)]
+    out = np.zeros(3, np.float32)
+    e = np.asarray(eta, np.float32)
+    kk = np.asarray(k, np.float32)
+    L.oracle_fresnel_conductor(cos_i, _fp(e), _fp(kk), _fp(out))
+    return out
+
+
+def ggx_D(ax, ay, wh):
+    L = lib()
+    L.oracle_ggx_D.restype = C.c_float
+    L.oracle_ggx_D.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float)]
+    w = np.ascontiguousarray(wh, np.float32)
+    return L.oracle_ggx_D(ax, ay, _fp(w))
+
+
+def env_sample(scene, u1, u2):
+    L = lib()
+    L.oracle_env_sample.argtypes = [C.c_void_p, C.c_float, C.c_float] + [C.POINTER(C.c_float)] * 3
+    d = np.zeros(3, np.float32)
+    p = np.zeros(1, np.float32)
+    q = np.zeros(1, np.float32)
+    L.oracle_env_sample(scene._h, u1, u2, _fp(d), _fp(p), _fp(q))
+    return d, float(p[0]), float(q[0])
