@@ -235,6 +235,9 @@ COMPAT_QUAD_PDF, COMPAT_CDF_N_PLUS_1, COMPAT_CLAMP, COMPAT_WASTED_DRAWS = 1, 2, 
 COMPAT_ALL = 15
 LIGHT_SKY_MASK = 0x20
 
+OK, ERR_INVALID_ARG, ERR_INVALID_PAYLOAD, ERR_HIP, ERR_OUT_OF_MEMORY = 0, 1, 2, 3, 4
+ERR_STACK_DEPTH, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_QUEUE_OVERFLOW = 5, 6, 7, 8
+
 STATUS = {0: "ok", 1: "invalid argument", 2: "invalid payload", 3: "HIP error", 4: "out of memory",
           5: "stack depth", 6: "no device", 7: "unsupported", 8: "queue overflow"}
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "noor_scene.h"
 
 using noor::Context;
 
@@ -615,6 +616,18 @@ size_t noor_abi_struct_size(const char* name) {
     if (n == "noor_stats") return sizeof(noor_stats);
     if (n == "noor_ray") return sizeof(noor_ray);
     if (n == "noor_hit") return sizeof(noor_hit);
+    if (n == "noor_transform") return sizeof(noor_transform);
+    if (n == "noor_hosek_state") return sizeof(noor_hosek_state);
+    if (n == "noor_point_light") return sizeof(noor_point_light);
+    if (n == "noor_spot_light") return sizeof(noor_spot_light);
+    if (n == "noor_distant_light") return sizeof(noor_distant_light);
+    if (n == "noor_texture_view") return sizeof(noor_texture_view);
+    if (n == "noor_kernel_time") return sizeof(noor_kernel_time);
+    if (n == "noor_mesh_desc") return sizeof(noor_mesh_desc);
+    if (n == "noor_instance_desc") return sizeof(noor_instance_desc);
+    if (n == "noor_bvh_params") return sizeof(noor_bvh_params);
+    if (n == "noor_scene_desc") return sizeof(noor_scene_desc);
+    if (n == "noor_build_stats") return sizeof(noor_build_stats);
     return 0;
 }
 
