@@ -11,9 +11,9 @@ counts come from the device counters of an untimed counting pass (they are
 deterministic: both renderers trace the same rays).
 
 `roofline`: the dominant kernel of the wavefront renderer, with its
-algorithmic bytes per launch (32 B per BVH node visit, 48 B per watertight
-triangle test, 48 B per instance entry, plus its per-ray queue and record
-traffic) divided by its average launch duration, measured with HIP events on
+algorithmic bytes per launch (112 B per wide BVH node visit: four child boxes
+and references, 48 B per watertight triangle test, 48 B per instance entry,
+plus its per-ray queue and record traffic) divided by its average launch duration, measured with HIP events on
 the renderer's own stream during an extra, event-timed step.
 
 `cpu_baseline`: the CPU restatement of the same path (the oracle, FP32) on
@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s + spp/s @1080p (wavefront vs megakernel); bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
+WIDE_NODE_BYTES = 112  # four child boxes (96 B) + four child references (16 B) per wide-node visit
 
 
 def log(msg):
@@ -43,7 +44,7 @@ def traversal_bytes(st, prefix, tri_test):
     inst = st[prefix + "instance_entries"]
     second = st[prefix + "tri_second_fetch"]
     tri_bytes = 48 * tris if tri_test == 0 else 32 * tris + 16 * second
-    return 32 * nodes + tri_bytes + 48 * inst
+    return WIDE_NODE_BYTES * nodes + tri_bytes + 48 * inst
 
 
 def cpu_baseline(wl, target_s):

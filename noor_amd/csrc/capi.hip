@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "wide_bvh.h"
 #include "noor_scene.h"
 
 using noor::Context;
@@ -170,7 +171,7 @@ noor_status apply_spec(Context* c, const noor_spec* spec) {
     c->S.mis = spec->mis ? 1 : 0;
     c->S.bias = spec->reflection_bias;
     c->S.world_radius = spec->world_radius;
-    if (spec->bvh_root_node != c->S.root) return fail(c, NOOR_ERR_INVALID_ARG, "spec root differs from payload root");
+    if (spec->bvh_root_node != c->bvh_root) return fail(c, NOOR_ERR_INVALID_ARG, "spec root differs from payload root");
     return NOOR_OK;
 }
 
@@ -285,29 +286,43 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(c, NOOR_ERR_HIP, "stream creation failed"));
 
-    // stack depth of the push-both traversal for this BVH
+    // walk of the binary BVH: detects out-of-range links and cycles
     std::vector<int32_t> memo(P->num_bvh_nodes, -1);
     bool bad = false;
-    c->stack_need = std::max(1u, stack_need(P, P->bvh_root_node, memo, 0, &bad));
+    stack_need(P, P->bvh_root_node, memo, 0, &bad);
     if (bad) return bail(fail(c, NOOR_ERR_INVALID_PAYLOAD, "BVH references out of range or cyclic"));
-    if (c->stack_need > 96) return bail(fail(c, NOOR_ERR_STACK_DEPTH, "BVH needs " + std::to_string(c->stack_need) +
-                                                                      " stack entries, limit 96"));
     noor::DevScene& S = c->S;
-    S.stack_depth = (int)c->stack_need;
-    S.root = P->bvh_root_node;
     S.num_tris = P->num_triangles;
+    c->bvh_root = P->bvh_root_node;
 
-    // triangles and BVH
-    const uint32_t nt = P->num_triangles;
+    // four-wide BVH for the traversal kernels
     {
-        // nodes interleaved so that one node is one 32-byte fetch
-        std::vector<noor_uint4> nodes((size_t)2 * P->num_bvh_nodes);
-        for (uint32_t i = 0; i < P->num_bvh_nodes; ++i) {
-            nodes[2 * (size_t)i] = P->bvh_min_right_start[i];
-            nodes[2 * (size_t)i + 1] = P->bvh_max_axis_count[i];
+        noor::WideBvh wb;
+        const std::string werr = noor::build_wide_bvh(P, wb);
+        if (!werr.empty()) return bail(fail(c, NOOR_ERR_UNSUPPORTED, "wide BVH: " + werr));
+        c->stack_need = wb.stack_need;
+        if (c->stack_need > 4096) return bail(fail(c, NOOR_ERR_STACK_DEPTH, "BVH needs " +
+                                                  std::to_string(c->stack_need) + " stack entries, limit 4096"));
+        S.stack_depth = (int)c->stack_need;
+        S.stack_lds = std::min<int>(S.stack_depth, c->stack_lds);
+        S.wroot = wb.root;
+        if ((st = dev_upload(c, (noor::WNode**)&S.wbvh, wb.nodes.data(), wb.nodes.size())) != NOOR_OK) return bail(st);
+        if ((st = dev_upload(c, (uint32_t**)&S.inst_root, wb.inst_root.data(), wb.inst_root.size())) != NOOR_OK)
+            return bail(st);
+        c->wide_nodes = wb.nodes.size();
+        int cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
+        S.gstack_lanes = (uint32_t)cus * 32u * 64u;
+        S.gstack = nullptr;
+        if (S.stack_depth > S.stack_lds) {
+            const size_t n = (size_t)(S.stack_depth - S.stack_lds) * S.gstack_lanes;
+            if ((st = dev_upload(c, (uint32_t**)&S.gstack, (const uint32_t*)nullptr, n)) != NOOR_OK) return bail(st);
         }
-        if ((st = dev_upload(c, (noor_uint4**)&S.bvh, nodes.data(), nodes.size())) != NOOR_OK) return bail(st);
     }
+
+    // triangles
+    const uint32_t nt = P->num_triangles;
     if ((st = dev_upload(c, (noor_uint4**)&S.wald0, P->wald_ax_nu_nv_nd, nt)) != NOOR_OK) return bail(st);
     if ((st = dev_upload(c, (noor_uint4**)&S.wald1, P->wald_bnu_bnv_au_av, nt)) != NOOR_OK) return bail(st);
     if ((st = dev_upload(c, (noor_uint4**)&S.wald2, P->wald_cnu_cnv_deg_mat, nt)) != NOOR_OK) return bail(st);

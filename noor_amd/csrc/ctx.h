@@ -99,7 +99,10 @@ struct Context {
     uint32_t wave_capacity = 0;   // 0: min(pixels per layer, 2^22)
     int wave_block = 64;          // lanes per block of the persistent traversal kernels
     KernelTimer timer;
-    uint32_t stack_need = 0;
+    uint32_t stack_need = 0;      // traversal stack entries the wide BVH can need
+    int stack_lds = 24;           // of which at most this many per lane live in LDS
+    size_t wide_nodes = 0;
+    uint32_t bvh_root = 0;        // the payload's binary root (the spec must name it)
     std::string last_error;
 };
 

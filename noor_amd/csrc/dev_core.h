@@ -13,6 +13,7 @@
 
 #include "../../include/noor_hip.h"
 #include "dev_math.h"
+#include "wide_bvh.h"
 
 namespace noor {
 
@@ -93,8 +94,10 @@ struct DevMat {
 };
 
 struct DevScene {
-    // binary BVH in the reference layout: two uint4 per node (noor/cuda/bvh.cuh:28-140)
-    const uint4* __restrict__ bvh;  // node i: bvh[2i] = min|right, bvh[2i+1] = max|flags
+    // four-wide BVH derived from the payload's binary BVH (wide_bvh.h): 8 uint4 per node
+    const uint4* __restrict__ wbvh;
+    const uint32_t* __restrict__ inst_root;  // per transform: reference of its BLAS root
+    uint32_t* gstack;                        // traversal stack entries beyond the LDS part
     // triangle records in leaf order
     const uint4* __restrict__ wald0;
     const uint4* __restrict__ wald1;
@@ -115,7 +118,8 @@ struct DevScene {
     const float* __restrict__ dfunc;
     const float* __restrict__ dcdf;
     const noor_hosek* __restrict__ hosek;
-    uint32_t root;
+    uint32_t wroot;          // reference of the root
+    uint32_t gstack_lanes;   // lanes the global stack part is laid out for
     uint32_t n_area, n_point, n_spot, n_distant;
     int32_t num_lights;
     uint32_t env_w, env_h;
@@ -123,7 +127,8 @@ struct DevScene {
     int32_t sky_enabled, skydome_type, bounces, rr, mis;
     int32_t rng_kind, tri_test;
     uint32_t compat;
-    int32_t stack_depth;
+    int32_t stack_depth;     // entries a traversal can need (exact bound for this BVH)
+    int32_t stack_lds;       // of which the first stack_lds live in LDS
     float bias, world_radius;
     uint32_t num_tris;
     noor_camera cam;
@@ -251,27 +256,42 @@ __device__ __forceinline__ Xf xf_from(const noor_transform& t) {
 // ---------------------------------------------------------------------------------------------
 // traversal
 
-// CudaBBox::intersect (noor/cuda/bbox.cuh:62-80); a box is entered when its
-// entry distance is <= tmax so that equal-t hits can be ordered by id.
-__device__ __forceinline__ bool box_hit(const Ray& r, uint4 mn, uint4 mx) {
-    const float lx = __uint_as_float(mn.x), ly = __uint_as_float(mn.y), lz = __uint_as_float(mn.z);
-    const float hx = __uint_as_float(mx.x), hy = __uint_as_float(mx.y), hz = __uint_as_float(mx.z);
-    const float bx0 = (r.pn & 1u) ? hx : lx, bx1 = (r.pn & 1u) ? lx : hx;
-    const float by0 = (r.pn & 2u) ? hy : ly, by1 = (r.pn & 2u) ? ly : hy;
-    const float bz0 = (r.pn & 4u) ? hz : lz, bz1 = (r.pn & 4u) ? lz : hz;
-    float tMin = (bx0 - r.o.x) * r.inv.x;
-    float tMax = (bx1 - r.o.x) * r.inv.x;
-    const float tyMin = (by0 - r.o.y) * r.inv.y;
-    const float tyMax = (by1 - r.o.y) * r.inv.y;
-    if (tMin > tyMax || tyMin > tMax) return false;
-    if (tyMin > tMin) tMin = tyMin;
-    if (tyMax < tMax) tMax = tyMax;
-    const float tzMin = (bz0 - r.o.z) * r.inv.z;
-    const float tzMax = (bz1 - r.o.z) * r.inv.z;
-    if (tMin > tzMax || tzMin > tMax) return false;
-    if (tzMin > tMin) tMin = tzMin;
-    if (tzMax < tMax) tMax = tzMax;
-    return (tMin <= r.tmax) && (tMax > 0.0f);
+// Conservative slab test of the four child boxes of a wide node
+// (CudaBBox::intersect, noor/cuda/bbox.cuh:62-80).  The exit distance and the
+// ray's tmax are widened by 2^-20 relative, more than the rounding of this
+// test and of the triangle tests, so a box holding a hit that a triangle test
+// accepts is always entered.  Which boxes a traversal visits, and in which
+// order, then cannot change the closest hit (ties are ordered by primitive
+// id): the wide BVH returns the hits of the binary BVH it was built from.
+// fmaxf/fminf ignore the NaN of an origin on the slab plane of a zero
+// direction component.  Bit `c` of the result is set when child c is hit;
+// tn[c] is its entry distance.
+constexpr float BOX_SLACK = 1.00000095367431640625f;  // 1 + 2^-20
+
+__device__ __forceinline__ uint32_t box4_hit(const Ray& r, const uint4& lx, const uint4& ly, const uint4& lz,
+                                             const uint4& hx, const uint4& hy, const uint4& hz, float tn[4]) {
+    const uint4 nx = (r.pn & 1u) ? hx : lx, fx = (r.pn & 1u) ? lx : hx;
+    const uint4 ny = (r.pn & 2u) ? hy : ly, fy = (r.pn & 2u) ? ly : hy;
+    const uint4 nz = (r.pn & 4u) ? hz : lz, fz = (r.pn & 4u) ? lz : hz;
+    const uint32_t NX[4] = {nx.x, nx.y, nx.z, nx.w}, FX[4] = {fx.x, fx.y, fx.z, fx.w};
+    const uint32_t NY[4] = {ny.x, ny.y, ny.z, ny.w}, FY[4] = {fy.x, fy.y, fy.z, fy.w};
+    const uint32_t NZ[4] = {nz.x, nz.y, nz.z, nz.w}, FZ[4] = {fz.x, fz.y, fz.z, fz.w};
+    const float tmaxs = r.tmax * BOX_SLACK;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float tx0 = (__uint_as_float(NX[c]) - r.o.x) * r.inv.x;
+        const float tx1 = (__uint_as_float(FX[c]) - r.o.x) * r.inv.x;
+        const float ty0 = (__uint_as_float(NY[c]) - r.o.y) * r.inv.y;
+        const float ty1 = (__uint_as_float(FY[c]) - r.o.y) * r.inv.y;
+        const float tz0 = (__uint_as_float(NZ[c]) - r.o.z) * r.inv.z;
+        const float tz1 = (__uint_as_float(FZ[c]) - r.o.z) * r.inv.z;
+        const float t0 = fmaxf(fmaxf(tx0, ty0), tz0);
+        const float t1 = fminf(fminf(tx1, ty1), tz1) * BOX_SLACK;
+        tn[c] = t0;
+        if (t0 <= t1 && t0 <= tmaxs && t1 >= 0.0f) mask |= 1u << c;
+    }
+    return mask;
 }
 
 struct Counters {
@@ -420,20 +440,68 @@ struct Hit {
     uint32_t prim, key, inst, kind;
 };
 
-// Per-lane traversal stack in LDS, lane-interleaved: entry d of lane l at
-// base[d * stride + l], so the 64 lanes of a wave hit 64 different banks.
-struct LdsStack {
-    uint32_t* base;
-    int stride;
+// Per-lane traversal stack.  Entries [0, K) live in LDS, lane-interleaved
+// (entry d of a lane at base[d * stride], so the 64 lanes of a wave use 64
+// banks); deeper entries, which few rays reach, live in device memory laid
+// out the same way over all lanes of the launch.  `depth` is the exact bound
+// of the BVH, so the stack cannot overflow.
+struct TravStack {
+    uint32_t* lds;
+    int lstride;
+    int K;
+    uint32_t* glob;
+    uint32_t gstride;
     int depth;
-    __device__ __forceinline__ uint32_t& at(int d) const { return base[d * stride]; }
+    __device__ __forceinline__ uint32_t& at(int d) const {
+        return d < K ? lds[d * lstride] : glob[(size_t)(d - K) * gstride];
+    }
 };
+// Stack of this lane; the launch must use at most S.gstack_lanes lanes.
+__device__ __forceinline__ TravStack make_stack(const DevScene& S, uint32_t* smem) {
+    TravStack t;
+    t.lds = smem + threadIdx.x;
+    t.lstride = (int)blockDim.x;
+    t.K = S.stack_lds;
+    t.glob = S.gstack + (blockIdx.x * blockDim.x + threadIdx.x);
+    t.gstride = S.gstack_lanes;
+    t.depth = S.stack_depth;
+    return t;
+}
 
-// Two-level closest hit (noor/cuda/bvh.cuh:142-191).  Ties in t are broken
-// by (prim_id, instance); light nodes win only when strictly closer.
+struct WideNode {
+    uint4 lx, ly, lz, hx, hy, hz, ref;
+};
+__device__ __forceinline__ WideNode load_wnode(const DevScene& S, uint32_t idx) {
+    const uint4* p = S.wbvh + (size_t)idx * 8;
+    WideNode n;
+    n.lx = p[0];
+    n.ly = p[1];
+    n.lz = p[2];
+    n.hx = p[3];
+    n.hy = p[4];
+    n.hz = p[5];
+    n.ref = p[6];
+    return n;
+}
+__device__ __forceinline__ uint32_t ref_of(const uint4& r, int i) {
+    return i == 0 ? r.x : (i == 1 ? r.y : (i == 2 ? r.z : r.w));
+}
+__device__ __forceinline__ void cswap(float& a, float& b, uint32_t& ra, uint32_t& rb) {
+    const bool s = b < a;
+    const float ta = s ? b : a, tb = s ? a : b;
+    const uint32_t ua = s ? rb : ra, ub = s ? ra : rb;
+    a = ta;
+    b = tb;
+    ra = ua;
+    rb = ub;
+}
+
+// Two-level closest hit (noor/cuda/bvh.cuh:142-191) over the wide BVH.  Ties
+// in t are broken by (prim_id, instance); light nodes win only when strictly
+// closer.  Hit children are followed nearest first.
 template <bool STATS>
-__device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const LdsStack& st, Counters& c,
-                            bool& overflow) {
+__device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const TravStack& st, Counters& c,
+                                            bool& overflow) {
     int sp = 0;
     Ray lray = ray;
     uint32_t ins_idx = 0;
@@ -444,44 +512,38 @@ __device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h,
     h.inst = 0xFFFFFFFFu;
     h.t = 0.0f;
     h.u = h.v = 0.0f;
-    st.at(sp++) = S.root;
-    while (sp > 0) {
-        const uint32_t ni = st.at(--sp);
-        const uint4 mn = S.bvh[2 * (size_t)ni];
-        const uint4 mx = S.bvh[2 * (size_t)ni + 1];
-        const uint32_t w = mx.w;
-        if (STATS) c.nodes++;
-        if (w & NOOR_MESH_NODE_MASK) lray = ray;
-        if (w & NOOR_LIGHT_NODE_MASK) {
-            if (!box_hit(ray, mn, mx)) continue;
-            const uint32_t li = w & NOOR_LIGHT_MASK;
-            Ray tr = ray;
-            if (light_node_intersect(S, li, tr) && tr.tmax < ray.tmax) {
-                ray.tmax = tr.tmax;
-                hit = true;
-                h.t = tr.tmax;
-                h.kind = 2;
-                h.inst = li;
-                h.key = 0xFFFFFFFFu;
-                h.prim = 0xFFFFFFFFu;
-                h.u = h.v = 0.0f;
+    uint32_t cur = S.wroot;
+    for (;;) {
+        if (cur & REF_TLAS) lray = ray;
+        const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
+        if (kind == REF_INNER) {
+            if (STATS) c.nodes++;
+            const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
+            float tn[4];
+            const uint32_t m = box4_hit(lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
+            if (m != 0u) {
+                float d0 = (m & 1u) ? tn[0] : K_INF, d1 = (m & 2u) ? tn[1] : K_INF;
+                float d2 = (m & 4u) ? tn[2] : K_INF, d3 = (m & 8u) ? tn[3] : K_INF;
+                uint32_t r0 = n.ref.x, r1 = n.ref.y, r2 = n.ref.z, r3 = n.ref.w;
+                cswap(d0, d1, r0, r1);
+                cswap(d2, d3, r2, r3);
+                cswap(d0, d2, r0, r2);
+                cswap(d1, d3, r1, r3);
+                cswap(d1, d2, r1, r2);
+                const int k = __popc(m);
+                if (sp + k - 1 > st.depth) {
+                    overflow = true;
+                    break;
+                }
+                if (k > 3) st.at(sp++) = r3;
+                if (k > 2) st.at(sp++) = r2;
+                if (k > 1) st.at(sp++) = r1;
+                cur = r0;
+                continue;
             }
-            continue;
-        }
-        if (!box_hit(lray, mn, mx)) continue;
-        if (w & NOOR_MESH_INSTANCE_MASK) {
-            ins_idx = w & NOOR_TRANS_MASK;
-            if (STATS) c.inst++;
-            const Xf T = xf_load(S.xf + 9 * (size_t)ins_idx);
-            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
-            if (sp >= st.depth) { overflow = true; break; }
-            st.at(sp++) = mn.w;
-        } else if (w & NOOR_MESH_LEAF_MASK) {
-            const uint32_t cnt = w & NOOR_COUNT_MASK;
-            if (sp + (int)cnt > st.depth) { overflow = true; break; }
-            for (uint32_t i = 1; i <= cnt; ++i) st.at(sp++) = ni + i;
-        } else if (w & NOOR_TRI_LEAF_MASK) {
-            const uint32_t start = mn.w, end = start + (w & NOOR_TRI_COUNT_MASK);
+        } else if (kind == REF_LEAF) {
+            const uint32_t start = cur & REF_LEAF_START_MASK;
+            const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
             for (uint32_t ti = start; ti < end; ++ti) {
                 float t, u, v;
                 if (!tri_test<STATS>(S, lray, ti, t, u, v, c)) continue;
@@ -505,69 +567,82 @@ __device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h,
                     h.kind = 1;
                 }
             }
+        } else if (kind == REF_INST) {
+            ins_idx = cur & REF_INDEX_MASK;
+            if (STATS) c.inst++;
+            const Xf T = xf_load(S.xf + 9 * (size_t)ins_idx);
+            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
+            cur = S.inst_root[ins_idx];
+            if (cur != REF_EMPTY) continue;
         } else {
-            if (sp + 2 > st.depth) { overflow = true; break; }
-            const uint32_t axis = w & NOOR_AXIS_MASK;
-            if (((lray.pn >> axis) & 1u)) {
-                st.at(sp++) = ni + 1;
-                st.at(sp++) = mn.w;
-            } else {
-                st.at(sp++) = mn.w;
-                st.at(sp++) = ni + 1;
+            const uint32_t li = cur & REF_INDEX_MASK;
+            Ray tr = ray;
+            if (light_node_intersect(S, li, tr) && tr.tmax < ray.tmax) {
+                ray.tmax = tr.tmax;
+                hit = true;
+                h.t = tr.tmax;
+                h.kind = 2;
+                h.inst = li;
+                h.key = 0xFFFFFFFFu;
+                h.prim = 0xFFFFFFFFu;
+                h.u = h.v = 0.0f;
             }
         }
+        if (sp == 0) break;
+        cur = st.at(--sp);
     }
     return hit;
 }
 
-// Any hit (noor/cuda/bvh.cuh:193-234).
+// Any hit (noor/cuda/bvh.cuh:193-234) over the wide BVH.
 template <bool STATS>
-__device__ __forceinline__ bool bvh_any(const DevScene& S, const Ray& ray, const LdsStack& st, Counters& c, bool& overflow) {
+__device__ __forceinline__ bool bvh_any(const DevScene& S, const Ray& ray, const TravStack& st, Counters& c,
+                                        bool& overflow) {
     int sp = 0;
     Ray lray = ray;
-    st.at(sp++) = S.root;
-    while (sp > 0) {
-        const uint32_t ni = st.at(--sp);
-        const uint4 mn = S.bvh[2 * (size_t)ni];
-        const uint4 mx = S.bvh[2 * (size_t)ni + 1];
-        const uint32_t w = mx.w;
-        if (STATS) c.nodes++;
-        if (w & NOOR_MESH_NODE_MASK) lray = ray;
-        if (w & NOOR_LIGHT_NODE_MASK) {
-            if (!box_hit(ray, mn, mx)) continue;
-            Ray tr = ray;
-            if (light_node_intersect(S, w & NOOR_LIGHT_MASK, tr)) return true;
-            continue;
-        }
-        if (!box_hit(lray, mn, mx)) continue;
-        if (w & NOOR_MESH_INSTANCE_MASK) {
-            const uint32_t ins = w & NOOR_TRANS_MASK;
-            if (STATS) c.inst++;
-            const Xf T = xf_load(S.xf + 9 * (size_t)ins);
-            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
-            if (sp >= st.depth) { overflow = true; return false; }
-            st.at(sp++) = mn.w;
-        } else if (w & NOOR_MESH_LEAF_MASK) {
-            const uint32_t cnt = w & NOOR_COUNT_MASK;
-            if (sp + (int)cnt > st.depth) { overflow = true; return false; }
-            for (uint32_t i = 1; i <= cnt; ++i) st.at(sp++) = ni + i;
-        } else if (w & NOOR_TRI_LEAF_MASK) {
-            const uint32_t start = mn.w, end = start + (w & NOOR_TRI_COUNT_MASK);
+    uint32_t cur = S.wroot;
+    for (;;) {
+        if (cur & REF_TLAS) lray = ray;
+        const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
+        if (kind == REF_INNER) {
+            if (STATS) c.nodes++;
+            const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
+            float tn[4];
+            uint32_t m = box4_hit(lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
+            if (m != 0u) {
+                const int k = __popc(m);
+                if (sp + k - 1 > st.depth) {
+                    overflow = true;
+                    return false;
+                }
+                cur = ref_of(n.ref, __ffs(m) - 1);
+                m &= m - 1u;
+                while (m) {
+                    st.at(sp++) = ref_of(n.ref, __ffs(m) - 1);
+                    m &= m - 1u;
+                }
+                continue;
+            }
+        } else if (kind == REF_LEAF) {
+            const uint32_t start = cur & REF_LEAF_START_MASK;
+            const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
             for (uint32_t ti = start; ti < end; ++ti) {
                 float t, u, v;
                 if (tri_test<STATS>(S, lray, ti, t, u, v, c)) return true;
             }
+        } else if (kind == REF_INST) {
+            const uint32_t ins = cur & REF_INDEX_MASK;
+            if (STATS) c.inst++;
+            const Xf T = xf_load(S.xf + 9 * (size_t)ins);
+            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
+            cur = S.inst_root[ins];
+            if (cur != REF_EMPTY) continue;
         } else {
-            if (sp + 2 > st.depth) { overflow = true; return false; }
-            const uint32_t axis = w & NOOR_AXIS_MASK;
-            if (((lray.pn >> axis) & 1u)) {
-                st.at(sp++) = ni + 1;
-                st.at(sp++) = mn.w;
-            } else {
-                st.at(sp++) = mn.w;
-                st.at(sp++) = ni + 1;
-            }
+            Ray tr = ray;
+            if (light_node_intersect(S, cur & REF_INDEX_MASK, tr)) return true;
         }
+        if (sp == 0) break;
+        cur = st.at(--sp);
     }
     return false;
 }

@@ -7,6 +7,8 @@
 // them; the per-frame running mean is applied in frame order, so the film is
 // identical to `spp` single-frame launches.  The traversal stack lives in LDS,
 // interleaved by lane.
+#include <algorithm>
+
 #include "ctx.h"
 
 namespace noor {
@@ -43,15 +45,15 @@ __global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ 
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t W = S.cam.w, H = S.cam.h;
-    const uint32_t tiles_x = (W + 7) / 8;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t x = (tile % tiles_x) * 8 + (lane & 7);
-    const uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
-    const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
+    const uint32_t tiles_x = (W + 7) / 8, tiles = tiles_x * ((H + 7) / 8);
+    const TravStack st = make_stack(S, smem);
     Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
     unsigned long long ext = 0, shd = 0;
     bool overflow = false;
-    if (x < W && y < H) {
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t x = (tile % tiles_x) * 8 + (lane & 7);
+        const uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
+        if (x >= W || y >= H) continue;
         const uint32_t gid = y * W + x;
         float4 acc = film[gid];
         f3 lookat = f3{0.0f, 0.0f, 0.0f};
@@ -97,14 +99,16 @@ hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp) {
     hipError_t se = sync_scene(c);
     if (se != hipSuccess) return se;
     const uint32_t tiles = ((c.S.cam.w + 7) / 8) * ((c.S.cam.h + 7) / 8);
-    const size_t lds = (size_t)64 * c.S.stack_depth * sizeof(uint32_t);
+    // one wave per 8x8 tile, grid-stride over tiles within the lanes the global stack covers
+    const uint32_t blocks = std::min<uint32_t>(tiles, c.S.gstack_lanes / 64);
+    const size_t lds = (size_t)64 * c.S.stack_lds * sizeof(uint32_t);
     KernelTimer& T = const_cast<KernelTimer&>(c.timer);
     T.begin(c.stream, "megakernel");
     if (c.opt.stats)
-        hipLaunchKernelGGL(k_megakernel<true>, dim3(tiles), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
+        hipLaunchKernelGGL(k_megakernel<true>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
                            c.d_stats, c.d_lookat);
     else
-        hipLaunchKernelGGL(k_megakernel<false>, dim3(tiles), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
+        hipLaunchKernelGGL(k_megakernel<false>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
                            c.d_stats, c.d_lookat);
     T.end(c.stream);
     return hipGetLastError();
@@ -116,12 +120,11 @@ __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, c
                                               int any_hit, noor_hit* __restrict__ hits, DevStats* stats) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
+    const TravStack st = make_stack(S, smem);
     Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
     bool overflow = false;
     unsigned long long ext = 0, shd = 0;
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const noor_ray r = rays[i];
         Ray ray = mkray(f3{r.o.x, r.o.y, r.o.z}, f3{r.d.x, r.d.y, r.d.z}, r.tmax);
         noor_hit out;
@@ -155,8 +158,8 @@ __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, c
 hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits) {
     hipError_t se = sync_scene(c);
     if (se != hipSuccess) return se;
-    const size_t lds = (size_t)64 * c.S.stack_depth * sizeof(uint32_t);
-    const uint32_t blocks = (n + 63) / 64;
+    const size_t lds = (size_t)64 * c.S.stack_lds * sizeof(uint32_t);
+    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((n + 63) / 64, c.S.gstack_lanes / 64));
     if (c.opt.stats)
         hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit, d_hits,
                            c.d_stats);

@@ -50,7 +50,7 @@ struct WaveCtl {
 struct WaveBuffers {
     uint32_t capacity = 0;
     int block = 64;
-    int stack_depth = 0;
+    int stack_lds = 0;
     // path state (slot = item index within the chunk)
     uint32_t* pix = nullptr;
     uint32_t* view = nullptr;
@@ -188,7 +188,7 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     WaveCtl* ctl = B.ctl;
-    const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
+    const TravStack st = make_stack(S, smem);
     const uint32_t count = ctl->ray_count[cur];
     const uint32_t* q = B.q_ray[cur];
     Counters cnt{0, 0, 0, 0};
@@ -303,7 +303,7 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     WaveCtl* ctl = B.ctl;
-    const LdsStack st{smem + threadIdx.x, (int)blockDim.x, S.stack_depth};
+    const TravStack st = make_stack(S, smem);
     const uint32_t count = ctl->shadow_count;
     Counters cnt{0, 0, 0, 0};
     bool overflow = false;
@@ -417,14 +417,9 @@ void wave_free(Context& c) {
     c.wave = nullptr;
 }
 
-static int persistent_blocks() {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-    }
-    return cus * 32;
-}
+// Persistent traversal grid: as many lanes as the global stack part covers
+// (32 waves per CU).
+static uint32_t persistent_lanes(const Context& c) { return c.S.gstack_lanes; }
 
 // Records one chunk of one layer (generate, bounces, finish, advance) on c.stream.
 static hipError_t record_chunk(Context& c, WaveBuffers& B, float4* films) {
@@ -432,7 +427,7 @@ static hipError_t record_chunk(Context& c, WaveBuffers& B, float4* films) {
     const uint32_t cap = B.capacity;
     const dim3 g1((cap + 255) / 256), b1(256);
     const int blk = B.block;
-    const size_t lds = (size_t)blk * S.stack_depth * sizeof(uint32_t);
+    const size_t lds = (size_t)blk * S.stack_lds * sizeof(uint32_t);
     const dim3 gp(B.graph_grid_persist / (blk / 64)), bp(blk);
     KernelTimer& T = c.timer;
     T.begin(c.stream, "wave_generate");
@@ -481,20 +476,20 @@ hipError_t wave_render_layers(Context& c, const std::vector<std::vector<ViewDesc
     if (capacity == 0) capacity = std::min<uint32_t>(max_items, 1u << 22);
     capacity = std::max<uint32_t>(64u, (capacity + 63) & ~63u);
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    const size_t lds = (size_t)block * c.S.stack_depth * sizeof(uint32_t);
+    const size_t lds = (size_t)block * c.S.stack_lds * sizeof(uint32_t);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (!c.wave || c.wave->capacity != capacity || c.wave->max_views < max_views || c.wave->block != block ||
-        c.wave->stack_depth != c.S.stack_depth) {
+        c.wave->stack_lds != c.S.stack_lds) {
         wave_free(c);
         c.wave = new WaveBuffers();
         c.wave->block = block;
-        c.wave->stack_depth = c.S.stack_depth;
+        c.wave->stack_lds = c.S.stack_lds;
         hipError_t e = wave_alloc(c, *c.wave, capacity, std::max<uint32_t>(max_views, 64));
         if (e != hipSuccess) {
             wave_free(c);
             return e;
         }
-        c.wave->graph_grid_persist = (uint32_t)persistent_blocks();
+        c.wave->graph_grid_persist = persistent_lanes(c) / 64u;
     }
     WaveBuffers& B = *c.wave;
     if (c.timer.enabled) use_graph = false;  // per-kernel events need individual launches

@@ -1,0 +1,282 @@
+// Host construction of the four-wide BVH from the payload's binary BVH.
+//
+// Binary node kinds follow the reference traversal (noor/cuda/bvh.cuh:142-191):
+// light node, mesh instance (BLAS root in .w of the first word), mesh leaf
+// (its instances follow at +1..+count), triangle leaf (start in .w, count in
+// the flags), else an inner node (left child at +1, right child in .w; MESH_INNER
+// marks the TLAS).  A wide node is formed by opening, largest surface area
+// first, inner children of the same level until four children are collected.
+// Boxes are copied bit for bit from the binary nodes.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+
+#include "wide_bvh.h"
+
+namespace noor {
+namespace {
+
+enum BinKind { B_LIGHT, B_INST, B_MLEAF, B_TLEAF, B_INNER };
+
+struct Child {
+    float lo[3], hi[3];
+    uint32_t ref;
+};
+
+class Builder {
+  public:
+    Builder(const noor_payload_view* P, WideBvh& W) : P_(P), W_(W) {}
+
+    std::string run() {
+        W_.nodes.clear();
+        W_.inst_root.assign(P_->num_instances, REF_EMPTY);
+        const Child root = make_child(P_->bvh_root_node);
+        if (!err_.empty()) return err_;
+        W_.root = root.ref;
+        if (W_.nodes.size() >= REF_INDEX_MASK) return "too many wide BVH nodes";
+        need_.assign(W_.nodes.size(), -1);
+        W_.stack_need = std::max<uint32_t>(1u, need(W_.root));
+        return err_;
+    }
+
+  private:
+    const noor_payload_view* P_;
+    WideBvh& W_;
+    std::string err_;
+    std::unordered_map<uint32_t, uint32_t> blas_;
+    std::vector<int32_t> need_;
+
+    uint32_t flags(uint32_t i) const { return P_->bvh_max_axis_count[i].w; }
+    uint32_t word(uint32_t i) const { return P_->bvh_min_right_start[i].w; }
+    BinKind kind(uint32_t i) const {
+        const uint32_t w = flags(i);
+        if (w & NOOR_LIGHT_NODE_MASK) return B_LIGHT;
+        if (w & NOOR_MESH_INSTANCE_MASK) return B_INST;
+        if (w & NOOR_MESH_LEAF_MASK) return B_MLEAF;
+        if (w & NOOR_TRI_LEAF_MASK) return B_TLEAF;
+        return B_INNER;
+    }
+    bool tlas(uint32_t i) const { return (flags(i) & NOOR_MESH_NODE_MASK) != 0; }
+    Child box_of(uint32_t i) const {
+        Child c;
+        const noor_uint4 a = P_->bvh_min_right_start[i], b = P_->bvh_max_axis_count[i];
+        const uint32_t lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+        for (int k = 0; k < 3; ++k) {
+            std::memcpy(&c.lo[k], &lo[k], 4);
+            std::memcpy(&c.hi[k], &hi[k], 4);
+        }
+        c.ref = REF_EMPTY;
+        return c;
+    }
+    static float area(const Child& c) {
+        const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+        if (!(dx >= 0.0f && dy >= 0.0f && dz >= 0.0f)) return 0.0f;
+        return dx * dy + dy * dz + dz * dx;
+    }
+    std::vector<uint32_t> expand(uint32_t i) const {
+        std::vector<uint32_t> ch;
+        if (kind(i) == B_MLEAF) {
+            const uint32_t n = flags(i) & NOOR_COUNT_MASK;
+            for (uint32_t j = 1; j <= n; ++j) ch.push_back(i + j);
+        } else {
+            ch.push_back(i + 1);
+            ch.push_back(word(i));
+        }
+        return ch;
+    }
+    bool openable(uint32_t i) const {
+        const BinKind k = kind(i);
+        return k == B_INNER || k == B_MLEAF;
+    }
+    size_t fanout(uint32_t i) const { return kind(i) == B_MLEAF ? (flags(i) & NOOR_COUNT_MASK) : 2u; }
+
+    uint32_t new_node() {
+        WNode n;
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 4; ++c) {
+                n.lo[k][c] = std::numeric_limits<float>::infinity();
+                n.hi[k][c] = -std::numeric_limits<float>::infinity();
+            }
+        for (int c = 0; c < 4; ++c) {
+            n.ref[c] = REF_EMPTY;
+            n.pad[c] = 0;
+        }
+        W_.nodes.push_back(n);
+        return (uint32_t)(W_.nodes.size() - 1);
+    }
+    void fill(uint32_t idx, const std::vector<Child>& cs) {
+        WNode& n = W_.nodes[idx];
+        for (size_t c = 0; c < cs.size(); ++c) {
+            for (int k = 0; k < 3; ++k) {
+                n.lo[k][c] = cs[c].lo[k];
+                n.hi[k][c] = cs[c].hi[k];
+            }
+            n.ref[c] = cs[c].ref;
+        }
+    }
+    // A wide node (or a small tree of them) over more than four children.
+    Child group(std::vector<Child> cs, bool is_tlas) {
+        while (cs.size() > 4) {
+            std::vector<Child> up;
+            const size_t per = (cs.size() + 3) / 4;
+            for (size_t s = 0; s < cs.size(); s += per) {
+                const size_t e = std::min(cs.size(), s + per);
+                std::vector<Child> part(cs.begin() + s, cs.begin() + e);
+                Child u = part[0];
+                for (const Child& p : part)
+                    for (int k = 0; k < 3; ++k) {
+                        u.lo[k] = std::min(u.lo[k], p.lo[k]);
+                        u.hi[k] = std::max(u.hi[k], p.hi[k]);
+                    }
+                const uint32_t idx = new_node();
+                fill(idx, part);
+                u.ref = (is_tlas ? REF_TLAS : 0u) | (REF_INNER << REF_KIND_SHIFT) | idx;
+                up.push_back(u);
+            }
+            cs.swap(up);
+        }
+        const uint32_t idx = new_node();
+        fill(idx, cs);
+        Child u = cs.empty() ? Child{} : cs[0];
+        for (const Child& p : cs)
+            for (int k = 0; k < 3; ++k) {
+                u.lo[k] = std::min(u.lo[k], p.lo[k]);
+                u.hi[k] = std::max(u.hi[k], p.hi[k]);
+            }
+        u.ref = (is_tlas ? REF_TLAS : 0u) | (REF_INNER << REF_KIND_SHIFT) | idx;
+        return u;
+    }
+
+    uint32_t blas_ref(uint32_t b) {
+        auto it = blas_.find(b);
+        if (it != blas_.end()) return it->second;
+        const BinKind k = kind(b);
+        if (k != B_INNER && k != B_TLEAF) {
+            err_ = "instance points at a node that is not a BLAS root";
+            return REF_EMPTY;
+        }
+        const uint32_t r = make_child(b).ref;
+        blas_[b] = r;
+        return r;
+    }
+
+    Child make_child(uint32_t i) {
+        Child c = box_of(i);
+        if (!err_.empty()) return c;
+        const uint32_t w = flags(i);
+        switch (kind(i)) {
+        case B_LIGHT:
+            c.ref = REF_TLAS | (REF_LIGHT << REF_KIND_SHIFT) | (w & NOOR_LIGHT_MASK);
+            return c;
+        case B_INST: {
+            const uint32_t ins = w & NOOR_TRANS_MASK;
+            const uint32_t r = blas_ref(word(i));
+            if (ins >= W_.inst_root.size()) {
+                err_ = "instance index out of range";
+                return c;
+            }
+            if (W_.inst_root[ins] != REF_EMPTY && W_.inst_root[ins] != r) {
+                err_ = "one transform used with two different BLAS roots";
+                return c;
+            }
+            W_.inst_root[ins] = r;
+            c.ref = REF_TLAS | (REF_INST << REF_KIND_SHIFT) | ins;
+            return c;
+        }
+        case B_TLEAF: {
+            const uint32_t start = word(i), count = w & NOOR_TRI_COUNT_MASK;
+            if (count == 0) return c;  // empty leaf: no reference
+            if ((uint64_t)start + count > (uint64_t)REF_LEAF_START_MASK + 1) {
+                err_ = "more than 67108864 triangles";
+                return c;
+            }
+            if (count <= REF_LEAF_MAX) {
+                c.ref = (REF_LEAF << REF_KIND_SHIFT) | ((count - 1) << REF_LEAF_COUNT_SHIFT) | start;
+                return c;
+            }
+            std::vector<Child> parts;
+            for (uint32_t s = 0; s < count; s += REF_LEAF_MAX) {
+                Child p = c;
+                const uint32_t n = std::min(REF_LEAF_MAX, count - s);
+                p.ref = (REF_LEAF << REF_KIND_SHIFT) | ((n - 1) << REF_LEAF_COUNT_SHIFT) | (start + s);
+                parts.push_back(p);
+            }
+            return group(parts, false);
+        }
+        case B_MLEAF:
+        case B_INNER:
+            break;
+        }
+        const bool is_tlas = tlas(i);
+        std::vector<uint32_t> ch = expand(i);
+        if (ch.size() > 4) {
+            std::vector<Child> cs;
+            for (uint32_t j : ch) {
+                Child cc = make_child(j);
+                if (cc.ref != REF_EMPTY) cs.push_back(cc);
+            }
+            return group(cs, is_tlas);
+        }
+        // open the largest inner children while the node has room
+        for (;;) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (size_t j = 0; j < ch.size(); ++j) {
+                if (!openable(ch[j]) || ch.size() - 1 + fanout(ch[j]) > 4) continue;
+                const float a = area(box_of(ch[j]));
+                if (a > best_area) {
+                    best_area = a;
+                    best = (int)j;
+                }
+            }
+            if (best < 0) break;
+            const std::vector<uint32_t> sub = expand(ch[best]);
+            ch.erase(ch.begin() + best);
+            ch.insert(ch.begin() + best, sub.begin(), sub.end());
+        }
+        const uint32_t idx = new_node();
+        std::vector<Child> cs;
+        for (uint32_t j : ch) {
+            Child cc = make_child(j);
+            if (cc.ref != REF_EMPTY) cs.push_back(cc);
+        }
+        fill(idx, cs);
+        c.ref = (is_tlas ? REF_TLAS : 0u) | (REF_INNER << REF_KIND_SHIFT) | idx;
+        return c;
+    }
+
+    // Most entries a traversal holds on its stack below `ref`: at a node with
+    // k children up to k - 1 are pushed while one is followed.
+    uint32_t need(uint32_t ref) {
+        if (ref == REF_EMPTY) return 0;
+        const uint32_t kind = (ref >> REF_KIND_SHIFT) & 3u;
+        if (kind == REF_INST) {
+            const uint32_t ins = ref & REF_INDEX_MASK;
+            return ins < W_.inst_root.size() ? need(W_.inst_root[ins]) : 0u;
+        }
+        if (kind != REF_INNER) return 0;
+        const uint32_t idx = ref & REF_INDEX_MASK;
+        if (need_[idx] >= 0) return (uint32_t)need_[idx];
+        const WNode n = W_.nodes[idx];
+        uint32_t k = 0, m = 0;
+        for (int c = 0; c < 4; ++c) {
+            if (n.ref[c] == REF_EMPTY) continue;
+            ++k;
+            m = std::max(m, need(n.ref[c]));
+        }
+        const uint32_t r = k ? (k - 1 + m) : 0u;
+        need_[idx] = (int32_t)r;
+        return r;
+    }
+};
+
+}  // namespace
+
+std::string build_wide_bvh(const noor_payload_view* P, WideBvh& out) {
+    Builder b(P, out);
+    return b.run();
+}
+
+}  // namespace noor

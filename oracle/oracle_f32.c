@@ -14,6 +14,8 @@
 #define R_SQRT sqrtf
 #define R_FLOOR floorf
 #define R_FABS fabsf
+#define R_FMAX fmaxf
+#define R_FMIN fminf
 #include "oracle_core.inc"
 
 /* ---- unit-test hooks into the FP32 integrator -------------------------------- */

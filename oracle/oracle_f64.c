@@ -13,4 +13,6 @@
 #define R_SQRT sqrt
 #define R_FLOOR floor
 #define R_FABS fabs
+#define R_FMAX fmax
+#define R_FMIN fmin
 #include "oracle_core.inc"

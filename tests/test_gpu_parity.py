@@ -4,11 +4,13 @@ Bit-exact for hit queries (t, barycentrics, primitive id, instance); images
 with identical per-pixel RNG streams within a mean relative error of 1e-4
 (the tolerance the specification states for floating point).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from noor_amd import abi, scenes
-from noor_amd.renderer import Renderer, hits_to_numpy
+from noor_amd.renderer import Renderer, hits_to_numpy, rays_from_numpy
 from oracle.binding import OracleScene
 from tests.helpers import mean_rel_err, random_rays, vertex_rays
 
@@ -51,6 +53,31 @@ def test_trace_any_bit_exact(cornell_small):
     g = hits_to_numpy(r.trace(rays, any_hit=True))
     h, _ = o.trace(rays, any_hit=True)
     np.testing.assert_array_equal(g[:, 6], hits_to_numpy(h)[:, 6])
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_trace_instanced_bit_exact(any_hit):
+    """Two-level scene (24 meshes under a TLAS): the GPU's four-wide BVH
+    returns the hits of the oracle's binary traversal of the payload BVH."""
+    wl = scenes.procedural(width=16, height=16, n_meshes=24, target_tris=60000)
+    r, o = _both(wl)
+    # rays aimed near the instances' world positions (the scene is sparse)
+    v = wl.payload.view
+    T = np.stack([np.ctypeslib.as_array(C.cast(v.object_to_world_row[k], C.POINTER(C.c_float)),
+                                        shape=(v.num_instances, 4))[:, 3] for k in range(3)], axis=1)
+    rng = np.random.default_rng(21)
+    tgt = T[rng.integers(0, v.num_instances, 30000)] + rng.normal(size=(30000, 3)) * 0.7
+    org = tgt + rng.normal(size=(30000, 3)) * 20
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = rays_from_numpy(org.astype(np.float32), d.astype(np.float32))
+    g = hits_to_numpy(r.trace(rays, any_hit=any_hit))
+    h = hits_to_numpy(o.trace(rays, any_hit=any_hit)[0])
+    if any_hit:
+        np.testing.assert_array_equal(g[:, 6], h[:, 6])
+    else:
+        assert (g[:, 6] > 0).sum() > 10000
+        np.testing.assert_array_equal(g, h)
 
 
 def test_trace_vertex_targeted_bit_exact(cornell_small):
