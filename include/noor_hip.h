@@ -312,6 +312,41 @@ noor_status noor_reset_kernel_times(noor_ctx* ctx);
  * block of the traversal kernels (64, 128 or 256). */
 noor_status noor_set_wavefront_params(noor_ctx* ctx, uint32_t queue_capacity, int32_t block);
 
+/* Traversal stack entries per lane kept in LDS (the rest of the exact
+ * per-BVH bound lives in device memory); 0 restores the default (24). */
+noor_status noor_set_stack_lds(noor_ctx* ctx, uint32_t entries);
+
+/* ---- multi-view batches (no counterpart in the reference, which rendered
+ * one camera per process) ----------------------------------------------- */
+typedef struct noor_view {
+    noor_camera camera;
+    uint32_t frame;   /* running-mean frame of the view's first sample (>= 1) */
+    uint32_t spp;     /* samples to add */
+    uint32_t seed;    /* RNG stream offset: sample s uses stream frame + s + seed */
+    uint32_t pad;
+} noor_view;
+/* Renders every view of the batch in one wavefront pass into films the
+ * context owns (one float4 film per view, in view order).  The films persist
+ * between calls with the same view sizes, so a later call with frame > 1
+ * continues each view's running mean; other sizes reallocate them (zeroed). */
+noor_status noor_render_batch(noor_ctx* ctx, const noor_view* views, uint32_t n);
+noor_status noor_read_batch_film(noor_ctx* ctx, uint32_t view, float* rgba, size_t count_floats);
+noor_status noor_batch_film_device_ptr(noor_ctx* ctx, uint32_t view, void** ptr);
+
+/* Wavefront autotuner: times noor_render_batch(views) for every setting of
+ * block {64, 128, 256} x queue capacity {2^20 .. 2^23} x LDS stack entries
+ * {16, 24, 32} (clamped to what the BVH needs), keeps the fastest on the
+ * context and returns the measurements (at most `max`).  The batch films are
+ * overwritten. */
+typedef struct noor_tune_entry {
+    uint32_t queue_capacity;
+    int32_t block;
+    uint32_t stack_lds;
+    float ms;
+} noor_tune_entry;
+noor_status noor_autotune(noor_ctx* ctx, const noor_view* views, uint32_t n, noor_tune_entry* out, uint32_t max,
+                          uint32_t* count);
+
 /* sizeof() of a named ABI struct as compiled into the library (layout checks). */
 size_t noor_abi_struct_size(const char* name);
 

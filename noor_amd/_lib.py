@@ -48,6 +48,12 @@ def lib():
         "noor_get_kernel_times": (C.c_int, [C.c_void_p, P(abi.KernelTime), C.c_uint32, P(C.c_uint32)]),
         "noor_reset_kernel_times": (C.c_int, [C.c_void_p]),
         "noor_set_wavefront_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32]),
+        "noor_set_stack_lds": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "noor_render_batch": (C.c_int, [C.c_void_p, P(abi.View), C.c_uint32]),
+        "noor_read_batch_film": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_float), C.c_size_t]),
+        "noor_batch_film_device_ptr": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p)]),
+        "noor_autotune": (C.c_int, [C.c_void_p, P(abi.View), C.c_uint32, P(abi.TuneEntry), C.c_uint32,
+                                    P(C.c_uint32)]),
         "noor_abi_struct_size": (C.c_size_t, [C.c_char_p]),
         "noor_scene_build": (C.c_int, [P(abi.SceneDesc), P(abi.BvhParams), P(C.c_void_p)]),
         "noor_payload_view_get": (C.c_int, [C.c_void_p, P(abi.PayloadView)]),

@@ -182,6 +182,14 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", c_u64), ("total_ms", C.c_double)]
 
 
+class View(C.Structure):
+    _fields_ = [("camera", Camera), ("frame", c_u32), ("spp", c_u32), ("seed", c_u32), ("pad", c_u32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("queue_capacity", c_u32), ("block", c_i32), ("stack_lds", c_u32), ("ms", c_f)]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [
         ("num_vertices", c_u32), ("positions", C.POINTER(c_f)), ("normals", C.POINTER(c_f)),

@@ -8,6 +8,7 @@
 //   - CUDA texture objects become raw pointers with software filtering;
 //   - every failure returns a status and records a message instead of exit().
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -210,6 +211,28 @@ noor_status apply_camera(Context* c, const noor_camera* cam) {
     return NOOR_OK;
 }
 
+// Splits the traversal stack: `k` entries per lane in LDS, the rest of the
+// BVH's exact bound in device memory (reallocated when the split changes).
+noor_status set_stack_lds(Context* c, int k) {
+    noor::DevScene& S = c->S;
+    k = std::max(1, std::min(k, S.stack_depth));
+    const size_t need = (size_t)(S.stack_depth - k) * S.gstack_lanes;
+    if (c->gstack) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipFree(c->gstack));
+        c->gstack = nullptr;
+    }
+    if (need) {
+        void* p = nullptr;
+        HIP_TRY(c, hipMalloc(&p, need * sizeof(uint32_t)));
+        c->gstack = (uint32_t*)p;
+    }
+    S.gstack = c->gstack;
+    S.stack_lds = k;
+    c->scene_dirty = true;
+    return NOOR_OK;
+}
+
 noor_status build_environment(Context* c) {
     c->scene_dirty = true;
     // PHYSICAL: bake sin(theta) * sky into a 2048^2 texture, then the CDF of it
@@ -314,11 +337,7 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
         S.gstack_lanes = (uint32_t)cus * 32u * 64u;
-        S.gstack = nullptr;
-        if (S.stack_depth > S.stack_lds) {
-            const size_t n = (size_t)(S.stack_depth - S.stack_lds) * S.gstack_lanes;
-            if ((st = dev_upload(c, (uint32_t**)&S.gstack, (const uint32_t*)nullptr, n)) != NOOR_OK) return bail(st);
-        }
+        if ((st = set_stack_lds(c, c->stack_lds)) != NOOR_OK) return bail(st);
     }
 
     // triangles
@@ -424,6 +443,8 @@ noor_status noor_destroy(noor_ctx* ctx) {
     noor::wave_free(*c);
     for (void* p : c->allocs) hipFree(p);
     if (c->film) hipFree(c->film);
+    if (c->gstack) hipFree(c->gstack);
+    if (c->batch_film) hipFree(c->batch_film);
     if (c->sky_bake) hipFree(c->sky_bake);
     if (c->dfunc) hipFree(c->dfunc);
     if (c->dcdf) hipFree(c->dcdf);
@@ -525,6 +546,145 @@ noor_status noor_render_wavefront(noor_ctx* ctx, uint32_t* frame, uint32_t spp) 
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->timer.collect();
     *frame += spp;
+    return NOOR_OK;
+}
+
+noor_status noor_set_stack_lds(noor_ctx* ctx, uint32_t entries) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    c->stack_lds = entries ? (int)std::min<uint32_t>(entries, 4096u) : 24;
+    return set_stack_lds(c, c->stack_lds);
+}
+
+namespace {
+noor_status check_views(Context* c, const noor_view* views, uint32_t n) {
+    if (!views || n == 0) return fail(c, NOOR_ERR_INVALID_ARG, "no views");
+    uint64_t pixels = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const noor_view& v = views[i];
+        if (v.camera.w == 0 || v.camera.h == 0 || v.camera.w > 16384 || v.camera.h > 16384)
+            return fail(c, NOOR_ERR_INVALID_ARG, "view " + std::to_string(i) + ": camera size");
+        if (v.camera.type < 0 || v.camera.type > 2)
+            return fail(c, NOOR_ERR_INVALID_ARG, "view " + std::to_string(i) + ": camera type");
+        if (v.frame < 1 || v.spp == 0) return fail(c, NOOR_ERR_INVALID_ARG, "view " + std::to_string(i) + ": frame/spp");
+        pixels += (uint64_t)v.camera.w * v.camera.h;
+    }
+    if (pixels >= (1ull << 31)) return fail(c, NOOR_ERR_INVALID_ARG, "batch larger than 2^31 pixels");
+    if (c->S.mis) return fail(c, NOOR_ERR_UNSUPPORTED, "the wavefront renderer does not implement MIS");
+    return NOOR_OK;
+}
+
+noor_status batch_films(Context* c, const noor_view* views, uint32_t n) {
+    bool same = c->batch_film && c->batch_w.size() == n;
+    for (uint32_t i = 0; same && i < n; ++i)
+        same = c->batch_w[i] == views[i].camera.w && c->batch_h[i] == views[i].camera.h;
+    if (same) return NOOR_OK;
+    if (c->batch_film) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipFree(c->batch_film));
+        c->batch_film = nullptr;
+    }
+    c->batch_w.resize(n);
+    c->batch_h.resize(n);
+    c->batch_off.resize(n);
+    size_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        c->batch_w[i] = views[i].camera.w;
+        c->batch_h[i] = views[i].camera.h;
+        c->batch_off[i] = off;
+        off += (size_t)views[i].camera.w * views[i].camera.h;
+    }
+    void* p = nullptr;
+    HIP_TRY(c, hipMalloc(&p, off * sizeof(float4)));
+    HIP_TRY(c, hipMemsetAsync(p, 0, off * sizeof(float4), c->stream));
+    c->batch_film = (float4*)p;
+    return NOOR_OK;
+}
+}  // namespace
+
+noor_status noor_render_batch(noor_ctx* ctx, const noor_view* views, uint32_t n) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    noor_status st = check_views(c, views, n);
+    if (st != NOOR_OK) return st;
+    if ((st = batch_films(c, views, n)) != NOOR_OK) return st;
+    HIP_TRY(c, noor::wave_render_batch(*c, views, n, c->batch_film, c->batch_off.data()));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timer.collect();
+    return NOOR_OK;
+}
+
+noor_status noor_read_batch_film(noor_ctx* ctx, uint32_t view, float* rgba, size_t count_floats) {
+    if (!ctx || !rgba) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    if (view >= c->batch_w.size()) return fail(c, NOOR_ERR_INVALID_ARG, "no such batch view");
+    const size_t need = (size_t)4 * c->batch_w[view] * c->batch_h[view];
+    if (count_floats < need) return fail(c, NOOR_ERR_INVALID_ARG, "buffer too small");
+    HIP_TRY(c, hipMemcpyAsync(rgba, c->batch_film + c->batch_off[view], need * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NOOR_OK;
+}
+
+noor_status noor_batch_film_device_ptr(noor_ctx* ctx, uint32_t view, void** ptr) {
+    if (!ctx || !ptr) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    if (view >= c->batch_w.size()) return fail(c, NOOR_ERR_INVALID_ARG, "no such batch view");
+    *ptr = c->batch_film + c->batch_off[view];
+    return NOOR_OK;
+}
+
+noor_status noor_autotune(noor_ctx* ctx, const noor_view* views, uint32_t n, noor_tune_entry* out, uint32_t max,
+                          uint32_t* count) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    noor_status st = check_views(c, views, n);
+    if (st != NOOR_OK) return st;
+    if ((st = batch_films(c, views, n)) != NOOR_OK) return st;
+    const int blocks[3] = {64, 128, 256};
+    const uint32_t caps[4] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    std::vector<int> ks;
+    for (int k : {16, 24, 32}) {
+        const int kk = std::min(k, c->S.stack_depth);
+        if (std::find(ks.begin(), ks.end(), kk) == ks.end()) ks.push_back(kk);
+    }
+    const bool timing = c->timer.enabled;
+    c->timer.enabled = false;
+    std::vector<noor_tune_entry> res;
+    noor_tune_entry best{0, 64, 24, 1e30f};
+    for (int k : ks) {
+        if ((st = set_stack_lds(c, k)) != NOOR_OK) break;
+        for (int b : blocks)
+            for (uint32_t cap : caps) {
+                c->wave_block = b;
+                c->wave_capacity = cap;
+                // first render allocates and captures; the second is timed
+                hipError_t e = noor::wave_render_batch(*c, views, n, c->batch_film, c->batch_off.data());
+                if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                const auto t0 = std::chrono::steady_clock::now();
+                if (e == hipSuccess) e = noor::wave_render_batch(*c, views, n, c->batch_film, c->batch_off.data());
+                if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                const auto t1 = std::chrono::steady_clock::now();
+                if (e != hipSuccess) {
+                    // a setting that does not fit (memory) is skipped
+                    (void)hipGetLastError();
+                    continue;
+                }
+                const float ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+                const noor_tune_entry r{cap, b, (uint32_t)k, ms};
+                res.push_back(r);
+                if (ms < best.ms) best = r;
+            }
+    }
+    c->timer.enabled = timing;
+    if (res.empty()) return fail(c, NOOR_ERR_HIP, "no wavefront setting could run");
+    c->wave_block = best.block;
+    c->wave_capacity = best.queue_capacity;
+    c->stack_lds = (int)best.stack_lds;
+    if ((st = set_stack_lds(c, c->stack_lds)) != NOOR_OK) return st;
+    if (count) *count = (uint32_t)std::min<size_t>(res.size(), max);
+    if (out)
+        for (uint32_t i = 0; i < std::min<size_t>(res.size(), max); ++i) out[i] = res[i];
     return NOOR_OK;
 }
 
@@ -643,6 +803,8 @@ size_t noor_abi_struct_size(const char* name) {
     if (n == "noor_bvh_params") return sizeof(noor_bvh_params);
     if (n == "noor_scene_desc") return sizeof(noor_scene_desc);
     if (n == "noor_build_stats") return sizeof(noor_build_stats);
+    if (n == "noor_view") return sizeof(noor_view);
+    if (n == "noor_tune_entry") return sizeof(noor_tune_entry);
     return 0;
 }
 

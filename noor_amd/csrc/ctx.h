@@ -103,6 +103,11 @@ struct Context {
     int stack_lds = 24;           // of which at most this many per lane live in LDS
     size_t wide_nodes = 0;
     uint32_t bvh_root = 0;        // the payload's binary root (the spec must name it)
+    uint32_t* gstack = nullptr;   // device part of the traversal stack
+    // multi-view batch films
+    float4* batch_film = nullptr;
+    std::vector<uint32_t> batch_w, batch_h;
+    std::vector<size_t> batch_off;  // float4 offset of each view's film
     std::string last_error;
 };
 
@@ -127,6 +132,7 @@ hipError_t launch_dist_build(const Context& c, const float4* tex, int tw, int th
                              int H);
 // kernels_wave.hip
 hipError_t wave_render(Context& c, uint32_t frame0, uint32_t spp);
+hipError_t wave_render_batch(Context& c, const noor_view* views, uint32_t n, float4* films, const size_t* offsets);
 void wave_free(Context& c);
 
 }  // namespace noor

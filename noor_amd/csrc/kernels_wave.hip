@@ -1,22 +1,27 @@
-// Wavefront renderer with material-sorted queues, persistent traversal
-// kernels and a hipGraph-captured bounce loop.
+// Wavefront renderer: streaming path regeneration, material-sorted shading
+// queues, persistent traversal kernels and hipGraph-captured iterations.
 //
-// One graph replay renders one chunk of one sample layer:
-//   k_generate  camera rays for the chunk's items (view, pixel)      -> ray queue
-//   per bounce:
-//     k_extend  closest hit (persistent waves pull 64-ray batches);
-//               each hit is binned by material class with one atomic
-//               per class per wave                                    -> class queues
-//     k_shade   walks the class queues in class order, so each wave
-//               shades one material; runs shade_bounce(), the same
-//               per-path arithmetic as the megakernel                 -> shadow queue, next ray queue
-//     k_shadow  any-hit for the next-event rays (persistent)         -> L
-//   k_finish    running mean into the film
-//   k_advance   bumps the device-side chunk index
-// Queue counters live in device memory and are cleared by the kernels
-// themselves (no memset nodes in the graph).  Because every path runs the
-// same functions on the same RNG stream as in the megakernel, the films of
-// the two renderers are identical.
+// A render is a list of sample items (sample, view, pixel), sample-major.  A
+// fixed pool of path slots is kept full; each iteration runs
+//   k_regen   retires the paths that ended in the previous iteration (their
+//             radiance goes to the item's entry of the sample buffer) and
+//             starts the next items in the freed slots           -> ray queue
+//   k_extend  closest hit; persistent waves pull 64-ray batches; each hit is
+//             binned by material class with one atomic per class per wave
+//   k_shade   walks the class queues in class order, so each wave shades one
+//             material; runs shade_bounce(), the per-bounce function of the
+//             megakernel: next rays -> ray queue, NEE rays -> shadow queue,
+//             ended paths -> free queue
+//   k_shadow  any hit for the NEE rays (persistent), resolved into L
+// Keeping the pool full means every launch has work for the whole GPU (a
+// layer-by-layer wavefront spends most of its time in nearly empty launches
+// of the late bounces).  When every item has retired, k_finish folds each
+// pixel's samples into its film in sample order with the running-mean update
+// (noor/cuda/framebuffer.cuh:63-75); every path runs the megakernel's
+// arithmetic on the same RNG stream, so the two renderers' films are
+// identical.  Queue counters are cleared by the kernels themselves (no memset
+// nodes in the graph); the host replays a graph of kIters iterations until
+// the device reports no live path.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -26,32 +31,50 @@
 namespace noor {
 
 constexpr int kClasses = 6;  // miss, emitter, lambert, conductor, dielectric, other
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+constexpr int kIters = 8;    // iterations per graph replay (even: each replay starts at parity 0)
+constexpr uint64_t kMaxGroupItems = 1ull << 27;  // sample-buffer entries per group of layers
 
 struct ViewDesc {
     noor_camera cam;
     uint32_t film_offset;  // float4 offset of this view's film
-    uint32_t frame;        // running-mean frame number of this layer
-    uint32_t seed;         // per-view RNG stream offset
-    uint32_t item_begin;   // first item of this view in the layer
+    uint32_t frame0;       // running-mean frame number of this view's first sample
+    uint32_t spp;          // samples of this view in this render
+    uint32_t seed;         // added to the frame number to form the RNG stream
+    uint32_t pix_begin;    // first pixel of this view in the finish enumeration
+    uint32_t pad[3];
+};
+
+// Items [item_begin, item_begin + w*h) of a group are sample `sample` of view `view`.
+struct Segment {
+    uint32_t item_begin;
+    uint32_t view;
+    uint32_t sample;
+    uint32_t pad;
 };
 
 struct WaveCtl {
     uint32_t ray_count[2];
+    uint32_t free_count[2];
     uint32_t class_count[kClasses];
     uint32_t shadow_count;
     uint32_t work_extend, work_shadow;
-    uint32_t chunk;          // index of the chunk being rendered
-    uint32_t layer_items;    // items in the layer
+    uint32_t item_next[2];  // next item to start, by iteration parity
+    uint32_t new_rays[2];   // rays k_regen appended after the continuing ones
+    uint32_t item_end;      // items of the current group
+    uint32_t num_segments;
     uint32_t num_views;
-    uint32_t capacity;
-    uint32_t pad[3];
+    uint32_t pad[1];
 };
 
 struct WaveBuffers {
     uint32_t capacity = 0;
     int block = 64;
     int stack_lds = 0;
-    // path state (slot = item index within the chunk)
+    uint32_t sb_items = 0;     // sample buffer entries
+    uint32_t max_views = 0, max_segments = 0;
+    // path state (one slot per path in flight)
+    uint32_t* item = nullptr;
     uint32_t* pix = nullptr;
     uint32_t* view = nullptr;
     float4* L = nullptr;       // L.xyz
@@ -67,19 +90,19 @@ struct WaveBuffers {
     float4* sh_o = nullptr;    // shadow ray origin, tmax
     float4* sh_d = nullptr;
     uint32_t* q_ray[2] = {nullptr, nullptr};
+    uint32_t* q_free[2] = {nullptr, nullptr};
     uint32_t* q_class = nullptr;  // kClasses * capacity
     uint32_t* q_shadow = nullptr;
+    float4* sb = nullptr;         // final radiance per item of the current group
     WaveCtl* ctl = nullptr;
     ViewDesc* views = nullptr;
-    uint32_t max_views = 0;
+    Segment* segs = nullptr;
+    int32_t* seg_of = nullptr;    // [sample * num_views + view] -> segment of the group, -1 if none
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int graph_bounces = -1;
-    DevScene graph_scene{};   // scene arguments baked into the captured graph
-    float4* graph_films = nullptr;
     bool graph_stats = false;
-    uint32_t graph_grid_persist = 0;
+    uint32_t persist_waves = 0;
     std::vector<void*> allocs;
 };
 
@@ -98,6 +121,38 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     base = __shfl(base, leader < 0 ? 0 : leader, 64);
     const unsigned long long below = mask & ((1ull << lane) - 1ull);
     return base + (uint32_t)__popcll(below);
+}
+
+// Block-aggregated appends to three queues at once (one atomic per queue per
+// block): lane results are this lane's slot in each queue where its flag is
+// set.  Every thread of the block must call it.
+__device__ __forceinline__ void block_append3(uint32_t* c0, bool f0, uint32_t* c1, bool f1, uint32_t* c2, bool f2,
+                                              uint32_t& s0, uint32_t& s1, uint32_t& s2) {
+    __shared__ uint32_t cnt[3][16];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1), m2 = __ballot(f2);
+    if (lane == 0) {
+        cnt[0][wave] = (uint32_t)__popcll(m0);
+        cnt[1][wave] = (uint32_t)__popcll(m1);
+        cnt[2][wave] = (uint32_t)__popcll(m2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint32_t* ctr = threadIdx.x == 0 ? c0 : (threadIdx.x == 1 ? c1 : c2);
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t n = cnt[threadIdx.x][w];
+            cnt[threadIdx.x][w] = tot;
+            tot += n;
+        }
+        const uint32_t base = tot ? atomicAdd(ctr, tot) : 0u;
+        for (uint32_t w = 0; w < nw; ++w) cnt[threadIdx.x][w] += base;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    s0 = cnt[0][wave] + (uint32_t)__popcll(m0 & below);
+    s1 = cnt[1][wave] + (uint32_t)__popcll(m1 & below);
+    s2 = cnt[2][wave] + (uint32_t)__popcll(m2 & below);
 }
 
 __device__ __forceinline__ void flush4(unsigned long long* dst, const Counters& c) {
@@ -139,48 +194,93 @@ __device__ __forceinline__ void store_state(const WaveBuffers& B, uint32_t p, co
 // ----------------------------------------------------------------------------------------------
 // kernels
 
-__global__ void k_generate(const DevScene* __restrict__ Sp, WaveBuffers B) {
+// Slot pool set-up for a group of `items` items.
+__global__ void k_stream_init(WaveBuffers B, uint32_t items) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    WaveCtl* ctl = B.ctl;
+    if (t < B.capacity) {
+        B.q_free[0][t] = t;
+        B.item[t] = kNoItem;
+    }
+    if (t == 0) {
+        ctl->ray_count[0] = ctl->ray_count[1] = 0;
+        ctl->free_count[0] = B.capacity;
+        ctl->free_count[1] = 0;
+        for (int c = 0; c < kClasses; ++c) ctl->class_count[c] = 0;
+        ctl->shadow_count = 0;
+        ctl->work_extend = ctl->work_shadow = 0;
+        ctl->item_next[0] = ctl->item_next[1] = 0;
+        ctl->new_rays[0] = ctl->new_rays[1] = 0;
+        ctl->item_end = items;
+    }
+}
+
+// Retires the paths of the free queue and starts new items in their slots.
+// Items are handed out in free-queue order, so no atomics are needed: thread
+// t starts item item_next + t and appends its ray after the continuing rays.
+__global__ void k_regen(const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
     const DevScene& S = *Sp;
     WaveCtl* ctl = B.ctl;
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t chunk = ctl->chunk;
-    const uint32_t begin = chunk * B.capacity;
-    const uint32_t items = ctl->layer_items;
-    const uint32_t n = (items > begin) ? min(B.capacity, items - begin) : 0u;
-    if (slot == 0) {
-        ctl->ray_count[0] = n;
-        ctl->ray_count[1] = 0;
+    const int nxt = cur ^ 1;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nfree = ctl->free_count[cur];
+    const uint32_t first = ctl->item_next[cur], end = ctl->item_end;
+    const uint32_t n_new = first < end ? min(nfree, end - first) : 0u;
+    const uint32_t c0 = ctl->ray_count[cur];
+    if (t == 0) {
+        ctl->item_next[nxt] = first + n_new;
+        ctl->new_rays[cur] = S.bounces > 0 ? n_new : 0u;
+        // counters the previous iteration's kernels consumed
+        ctl->free_count[nxt] = 0;
+        ctl->ray_count[nxt] = 0;
         for (int c = 0; c < kClasses; ++c) ctl->class_count[c] = 0;
         ctl->shadow_count = 0;
         ctl->work_extend = 0;
         ctl->work_shadow = 0;
     }
-    if (slot >= n) return;
-    const uint32_t item = begin + slot;
-    // locate the view of this item (views sorted by item_begin)
-    uint32_t lo = 0, hi = ctl->num_views - 1;
+    if (t >= nfree) return;
+    const uint32_t p = B.q_free[cur][t];
+    const uint32_t it = B.item[p];
+    if (it != kNoItem) {
+        const float4 L = B.L[p];
+        const f3 c = final_radiance(S, f3{L.x, L.y, L.z});
+        B.sb[it] = make_float4(c.x, c.y, c.z, 0.0f);
+        B.item[p] = kNoItem;
+    }
+    if (t >= n_new) return;
+    const uint32_t item = first + t;
+    if (S.bounces <= 0) {
+        // no bounce to trace: the sample is the (clamped) zero radiance
+        const f3 c = final_radiance(S, f3{0.0f, 0.0f, 0.0f});
+        B.sb[item] = make_float4(c.x, c.y, c.z, 0.0f);
+        return;
+    }
+    // segment of this item: segments are sorted by item_begin
+    uint32_t lo = 0, hi = ctl->num_segments - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
-        if (B.views[mid].item_begin <= item) lo = mid;
+        if (B.segs[mid].item_begin <= item) lo = mid;
         else hi = mid - 1;
     }
-    const ViewDesc& V = B.views[lo];
-    const uint32_t gid = item - V.item_begin;
+    const Segment sg = B.segs[lo];
+    const ViewDesc& V = B.views[sg.view];
+    const uint32_t gid = item - sg.item_begin;
     const uint32_t x = gid % V.cam.w, y = gid / V.cam.w;
     PathState ps;
     ps.L = f3{0.0f, 0.0f, 0.0f};
     ps.beta = f3{1.0f, 1.0f, 1.0f};
     ps.bounce = 0;
     ps.specular = 0;
-    rng_init(ps.rng, S.rng_kind, gid, V.frame + V.seed);
+    rng_init(ps.rng, S.rng_kind, gid, V.frame0 + sg.sample + V.seed);
     const Ray r = gen_ray(V.cam, x, y, ps.rng, S.rng_kind);
-    B.pix[slot] = gid;
-    B.view[slot] = lo;
-    B.rng_inc[slot] = ps.rng.inc;
-    store_state(B, slot, ps);
-    B.ro[slot] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-    B.rd[slot] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
-    B.q_ray[0][slot] = slot;
+    B.item[p] = item;
+    B.pix[p] = gid;
+    B.view[p] = sg.view;
+    B.rng_inc[p] = ps.rng.inc;
+    store_state(B, p, ps);
+    B.ro[p] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+    B.rd[p] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+    B.q_ray[cur][c0 + t] = p;
 }
 
 template <bool STATS>
@@ -189,7 +289,7 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
     extern __shared__ uint32_t smem[];
     WaveCtl* ctl = B.ctl;
     const TravStack st = make_stack(S, smem);
-    const uint32_t count = ctl->ray_count[cur];
+    const uint32_t count = ctl->ray_count[cur] + ctl->new_rays[cur];
     const uint32_t* q = B.q_ray[cur];
     Counters cnt{0, 0, 0, 0};
     bool overflow = false;
@@ -220,13 +320,8 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
             if (mine) B.q_class[(size_t)c * B.capacity + slot] = p;
         }
     }
-    // the shadow work counter of the previous bounce is no longer in use
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->shadow_count = 0;
-    const unsigned long long e = [&] {
-        unsigned long long v = ext;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        return v;
-    }();
+    unsigned long long e = ext;
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
     if (lane_id() == 0 && e) atomicAdd(&stats->extension_rays, e);
     if (STATS) flush4(stats->ext, cnt);
     const unsigned long long ov = __ballot(overflow);
@@ -249,12 +344,9 @@ __global__ void k_shade(const DevScene* __restrict__ Sp, WaveBuffers B, int cur)
         }
         prefix += n;
     }
-    if (t == 0) {
-        ctl->work_extend = 0;
-        ctl->ray_count[cur] = 0;
-    }
+    if (blockIdx.x * blockDim.x >= prefix) return;  // whole block idle (uniform per block)
     const int nxt = cur ^ 1;
-    bool cont = false, shadow = false;
+    bool cont = false, shadow = false, ended = false;
     uint32_t p = 0;
     if (cls >= 0) {
         p = B.q_class[(size_t)cls * B.capacity + local];
@@ -276,6 +368,9 @@ __global__ void k_shade(const DevScene* __restrict__ Sp, WaveBuffers B, int cur)
         if (hit) ray.tmax = h.t;
         Nee nee;
         cont = shade_bounce(S, ps, ray, hit, h, nee, nullptr);
+        // the megakernel traces at most `bounces` extension rays per path
+        if (cont && ps.bounce + 1 >= (uint32_t)S.bounces) cont = false;
+        ended = !cont;
         if (nee.has && nee.na) {
             shadow = true;
             B.nee_a[p] = make_float4(nee.a.x, nee.a.y, nee.a.z, 0.0f);
@@ -292,10 +387,11 @@ __global__ void k_shade(const DevScene* __restrict__ Sp, WaveBuffers B, int cur)
         }
         store_state(B, p, ps);
     }
-    const uint32_t s1 = wave_append(&ctl->shadow_count, shadow);
+    uint32_t s1, s2, s3;
+    block_append3(&ctl->shadow_count, shadow, &ctl->ray_count[nxt], cont, &ctl->free_count[nxt], ended, s1, s2, s3);
     if (shadow) B.q_shadow[s1] = p;
-    const uint32_t s2 = wave_append(&ctl->ray_count[nxt], cont);
     if (cont) B.q_ray[nxt][s2] = p;
+    if (ended) B.q_free[nxt][s3] = p;
 }
 
 template <bool STATS>
@@ -334,9 +430,6 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
             B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int c = 0; c < kClasses; ++c) ctl->class_count[c] = 0;
-    }
     unsigned long long v = shd;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane_id() == 0 && v) atomicAdd(&stats->shadow_rays, v);
@@ -345,33 +438,37 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
     if (lane_id() == 0 && ov) atomicAdd(&stats->overflow, 1ull);
 }
 
-// after the shadow kernel: reset the shadow work counter for the next bounce
-__global__ void k_between(WaveBuffers B) {
-    if (threadIdx.x == 0) B.ctl->work_shadow = 0;
-}
-
-__global__ void k_finish(const DevScene* __restrict__ Sp, WaveBuffers B, float4* films) {
-    const DevScene& S = *Sp;
-    WaveCtl* ctl = B.ctl;
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t begin = ctl->chunk * B.capacity;
-    const uint32_t items = ctl->layer_items;
-    const uint32_t n = (items > begin) ? min(B.capacity, items - begin) : 0u;
-    if (slot >= n) return;
-    const ViewDesc& V = B.views[B.view[slot]];
-    const float4 L = B.L[slot];
-    float4* f = films + V.film_offset + B.pix[slot];
-    *f = film_lerp(*f, final_radiance(S, f3{L.x, L.y, L.z}), V.frame);
-}
-
-__global__ void k_advance(WaveBuffers B) {
-    if (threadIdx.x == 0) B.ctl->chunk += 1;
+// Folds samples [s0, s1) of every pixel into its film, in sample order.
+__global__ void k_finish(WaveBuffers B, float4* films, uint32_t s0, uint32_t s1, uint32_t pixels) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pixels) return;
+    const WaveCtl* ctl = B.ctl;
+    const uint32_t nv = ctl->num_views;
+    uint32_t lo = 0, hi = nv - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (B.views[mid].pix_begin <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const ViewDesc& V = B.views[lo];
+    const uint32_t px = t - V.pix_begin;
+    const uint32_t end = min(s1, V.spp);
+    if (s0 >= end) return;
+    float4* f = films + V.film_offset + px;
+    float4 acc = *f;
+    for (uint32_t s = s0; s < end; ++s) {
+        const int32_t sg = B.seg_of[(size_t)s * nv + lo];
+        const float4 c = B.sb[B.segs[sg].item_begin + px];
+        acc = film_lerp(acc, f3{c.x, c.y, c.z}, V.frame0 + s);
+    }
+    *f = acc;
 }
 
 // ----------------------------------------------------------------------------------------------
 // host side
 
-static hipError_t wave_alloc(Context& c, WaveBuffers& B, uint32_t cap, uint32_t max_views) {
+static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, uint32_t max_views,
+                             uint32_t max_segments) {
     auto alloc = [&](void** p, size_t bytes) -> hipError_t {
         hipError_t e = hipMalloc(p, bytes);
         if (e == hipSuccess) B.allocs.push_back(*p);
@@ -380,6 +477,7 @@ static hipError_t wave_alloc(Context& c, WaveBuffers& B, uint32_t cap, uint32_t 
     hipError_t e = hipSuccess;
 #define WA(ptr, n) \
     if (e == hipSuccess) e = alloc((void**)&(ptr), sizeof(*(ptr)) * (size_t)(n));
+    WA(B.item, cap);
     WA(B.pix, cap);
     WA(B.view, cap);
     WA(B.L, cap);
@@ -396,14 +494,20 @@ static hipError_t wave_alloc(Context& c, WaveBuffers& B, uint32_t cap, uint32_t 
     WA(B.sh_d, cap);
     WA(B.q_ray[0], cap);
     WA(B.q_ray[1], cap);
+    WA(B.q_free[0], cap);
+    WA(B.q_free[1], cap);
     WA(B.q_class, (size_t)kClasses * cap);
     WA(B.q_shadow, cap);
+    WA(B.sb, sb_items);
     WA(B.ctl, 1);
     WA(B.views, max_views);
+    WA(B.segs, max_segments);
+    WA(B.seg_of, max_segments);
 #undef WA
     B.capacity = cap;
+    B.sb_items = sb_items;
     B.max_views = max_views;
-    (void)c;
+    B.max_segments = max_segments;
     return e;
 }
 
@@ -417,24 +521,19 @@ void wave_free(Context& c) {
     c.wave = nullptr;
 }
 
-// Persistent traversal grid: as many lanes as the global stack part covers
-// (32 waves per CU).
-static uint32_t persistent_lanes(const Context& c) { return c.S.gstack_lanes; }
-
-// Records one chunk of one layer (generate, bounces, finish, advance) on c.stream.
-static hipError_t record_chunk(Context& c, WaveBuffers& B, float4* films) {
-    const DevScene& S = c.S;
+// Records `iters` iterations (regen, extend, shade, shadow) on c.stream.
+static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
     const uint32_t cap = B.capacity;
     const dim3 g1((cap + 255) / 256), b1(256);
     const int blk = B.block;
-    const size_t lds = (size_t)blk * S.stack_lds * sizeof(uint32_t);
-    const dim3 gp(B.graph_grid_persist / (blk / 64)), bp(blk);
+    const size_t lds = (size_t)blk * B.stack_lds * sizeof(uint32_t);
+    const dim3 gp(B.persist_waves / (blk / 64)), bp(blk);
     KernelTimer& T = c.timer;
-    T.begin(c.stream, "wave_generate");
-    hipLaunchKernelGGL(k_generate, g1, b1, 0, c.stream, c.d_scene, B);
-    T.end(c.stream);
-    for (int b = 0; b < S.bounces; ++b) {
-        const int cur = b & 1;
+    for (int i = 0; i < iters; ++i) {
+        const int cur = i & 1;
+        T.begin(c.stream, "wave_regen");
+        hipLaunchKernelGGL(k_regen, g1, b1, 0, c.stream, c.d_scene, B, cur);
+        T.end(c.stream);
         T.begin(c.stream, "wave_extend");
         if (c.opt.stats)
             hipLaunchKernelGGL(k_extend<true>, gp, bp, lds, c.stream, c.d_scene, B, cur, c.d_stats);
@@ -450,112 +549,160 @@ static hipError_t record_chunk(Context& c, WaveBuffers& B, float4* films) {
         else
             hipLaunchKernelGGL(k_shadow<false>, gp, bp, lds, c.stream, c.d_scene, B, c.d_stats);
         T.end(c.stream);
-        hipLaunchKernelGGL(k_between, dim3(1), dim3(64), 0, c.stream, B);
     }
-    T.begin(c.stream, "wave_finish");
-    hipLaunchKernelGGL(k_finish, g1, b1, 0, c.stream, c.d_scene, B, films);
-    T.end(c.stream);
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c.stream, B);
     return hipGetLastError();
 }
 
-// Renders the given layers.  `layers[l]` lists the views active in layer l
-// (with their frame numbers); items of a layer are processed in chunks of
-// the queue capacity.  Uses a captured graph per chunk when use_graph.
-hipError_t wave_render_layers(Context& c, const std::vector<std::vector<ViewDesc>>& layers, float4* films,
-                              uint32_t capacity, int block, bool use_graph) {
-    hipError_t se = sync_scene(c);
-    if (se != hipSuccess) return se;
-    uint32_t max_items = 0, max_views = 1;
-    for (const auto& L : layers) {
-        uint32_t n = 0;
-        for (const auto& v : L) n += v.cam.w * v.cam.h;
-        max_items = std::max(max_items, n);
-        max_views = std::max<uint32_t>(max_views, (uint32_t)L.size());
-    }
-    if (capacity == 0) capacity = std::min<uint32_t>(max_items, 1u << 22);
-    capacity = std::max<uint32_t>(64u, (capacity + 63) & ~63u);
+// Renders `views` (each with its own camera, film, first frame and sample
+// count) into `films`.  Items are processed in groups of whole sample layers
+// so that a group's sample buffer holds at most kMaxGroupItems entries.
+hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, float4* films, uint32_t capacity,
+                             int block, bool use_graph) {
+    hipError_t e = sync_scene(c);
+    if (e != hipSuccess) return e;
+    if (views_in.empty()) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     const size_t lds = (size_t)block * c.S.stack_lds * sizeof(uint32_t);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    if (!c.wave || c.wave->capacity != capacity || c.wave->max_views < max_views || c.wave->block != block ||
-        c.wave->stack_lds != c.S.stack_lds) {
+    std::vector<ViewDesc> views = views_in;
+    const uint32_t nv = (uint32_t)views.size();
+    uint32_t max_spp = 0;
+    uint64_t pixels = 0;
+    for (auto& v : views) {
+        v.pix_begin = (uint32_t)pixels;
+        pixels += (uint64_t)v.cam.w * v.cam.h;
+        max_spp = std::max(max_spp, v.spp);
+    }
+    if (pixels >= (1ull << 31) || (uint64_t)max_spp * nv >= (1ull << 31)) return hipErrorInvalidValue;
+    // items per sample layer, and groups of whole layers within the sample buffer
+    std::vector<uint64_t> layer_items(max_spp, 0);
+    for (uint32_t s = 0; s < max_spp; ++s)
+        for (const auto& v : views)
+            if (v.spp > s) layer_items[s] += (uint64_t)v.cam.w * v.cam.h;
+    std::vector<std::pair<uint32_t, uint32_t>> groups;  // sample ranges [s0, s1)
+    uint64_t sb_need = 1;
+    for (uint32_t s = 0; s < max_spp;) {
+        uint64_t n = layer_items[s];
+        if (n > kMaxGroupItems) return hipErrorInvalidValue;
+        uint32_t e1 = s + 1;
+        while (e1 < max_spp && n + layer_items[e1] <= kMaxGroupItems) n += layer_items[e1++];
+        groups.push_back({s, e1});
+        sb_need = std::max(sb_need, n);
+        s = e1;
+    }
+    if (capacity == 0) capacity = (uint32_t)std::min<uint64_t>(sb_need, 1u << 22);
+    capacity = std::max<uint32_t>(256u, (capacity + 255) & ~255u);
+    const uint32_t seg_need = (uint32_t)std::max<uint64_t>((uint64_t)max_spp * nv, 1);
+    if (!c.wave || c.wave->capacity != capacity || c.wave->sb_items < sb_need || c.wave->max_views < nv ||
+        c.wave->max_segments < seg_need || c.wave->block != block || c.wave->stack_lds != c.S.stack_lds) {
         wave_free(c);
         c.wave = new WaveBuffers();
         c.wave->block = block;
         c.wave->stack_lds = c.S.stack_lds;
-        hipError_t e = wave_alloc(c, *c.wave, capacity, std::max<uint32_t>(max_views, 64));
+        e = wave_alloc(*c.wave, capacity, (uint32_t)sb_need, std::max<uint32_t>(nv, 64), std::max<uint32_t>(seg_need, 256));
         if (e != hipSuccess) {
             wave_free(c);
             return e;
         }
-        c.wave->graph_grid_persist = persistent_lanes(c) / 64u;
+        c.wave->persist_waves = c.S.gstack_lanes / 64u;
     }
     WaveBuffers& B = *c.wave;
     if (c.timer.enabled) use_graph = false;  // per-kernel events need individual launches
-    if (use_graph && B.exec &&
-        (B.graph_bounces != c.S.bounces || B.graph_films != films || B.graph_stats != (c.opt.stats != 0) ||
-         std::memcmp(&B.graph_scene, &c.S, sizeof(DevScene)) != 0)) {
-        if (B.exec) hipGraphExecDestroy(B.exec);
-        if (B.graph) hipGraphDestroy(B.graph);
+    if (B.exec && B.graph_stats != (c.opt.stats != 0)) {
+        hipGraphExecDestroy(B.exec);
+        hipGraphDestroy(B.graph);
         B.exec = nullptr;
         B.graph = nullptr;
     }
-    for (const auto& L : layers) {
-        if (L.empty()) continue;
-        std::vector<ViewDesc> vd = L;
+    e = hipMemcpyAsync(B.views, views.data(), sizeof(ViewDesc) * nv, hipMemcpyHostToDevice, c.stream);
+    if (e != hipSuccess) return e;
+    for (const auto& g : groups) {
+        // segments of this group in item order (sample-major, then view)
+        std::vector<Segment> segs;
+        std::vector<int32_t> seg_of((size_t)max_spp * nv, -1);
         uint32_t items = 0;
-        for (auto& v : vd) {
-            v.item_begin = items;
-            items += v.cam.w * v.cam.h;
-        }
+        for (uint32_t s = g.first; s < g.second; ++s)
+            for (uint32_t v = 0; v < nv; ++v) {
+                if (views[v].spp <= s) continue;
+                seg_of[(size_t)s * nv + v] = (int32_t)segs.size();
+                segs.push_back(Segment{items, v, s, 0});
+                items += views[v].cam.w * views[v].cam.h;
+            }
         WaveCtl ctl{};
-        ctl.chunk = 0;
-        ctl.layer_items = items;
-        ctl.num_views = (uint32_t)vd.size();
-        ctl.capacity = B.capacity;
-        hipError_t e = hipMemcpyAsync(B.views, vd.data(), sizeof(ViewDesc) * vd.size(), hipMemcpyHostToDevice, c.stream);
+        ctl.num_segments = (uint32_t)segs.size();
+        ctl.num_views = nv;
+        e = hipMemcpyAsync(B.segs, segs.data(), sizeof(Segment) * segs.size(), hipMemcpyHostToDevice, c.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(B.seg_of, seg_of.data(), sizeof(int32_t) * seg_of.size(), hipMemcpyHostToDevice,
+                               c.stream);
         if (e == hipSuccess) e = hipMemcpyAsync(B.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, c.stream);
         if (e != hipSuccess) return e;
-        const uint32_t chunks = (items + B.capacity - 1) / B.capacity;
-        for (uint32_t ch = 0; ch < chunks; ++ch) {
+        hipLaunchKernelGGL(k_stream_init, dim3((B.capacity + 255) / 256), dim3(256), 0, c.stream, B, items);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // Every live path retires within `bounces` iterations, so within that
+        // many iterations the whole pool is started afresh: the bound below is
+        // never reached unless the device state is corrupt.
+        const uint64_t max_iters =
+            2ull * ((uint64_t)items / B.capacity + 2) * (uint64_t)(std::max(c.S.bounces, 1) + 1) + 4 * kIters;
+        for (uint64_t it = 0;; it += kIters) {
             if (use_graph) {
                 if (!B.exec) {
                     e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
                     if (e != hipSuccess) return e;
-                    hipError_t er = record_chunk(c, B, films);
+                    hipError_t er = record_iterations(c, B, kIters);
                     e = hipStreamEndCapture(c.stream, &B.graph);
                     if (er != hipSuccess) return er;
                     if (e != hipSuccess) return e;
                     e = hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0);
                     if (e != hipSuccess) return e;
-                    B.graph_bounces = c.S.bounces;
-                    B.graph_scene = c.S;
-                    B.graph_films = films;
                     B.graph_stats = c.opt.stats != 0;
                 }
                 e = hipGraphLaunch(B.exec, c.stream);
             } else {
-                e = record_chunk(c, B, films);
+                e = record_iterations(c, B, kIters);
             }
             if (e != hipSuccess) return e;
+            WaveCtl h{};
+            e = hipMemcpyAsync(&h, B.ctl, sizeof(h), hipMemcpyDeviceToHost, c.stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+            if (e != hipSuccess) return e;
+            // the last iteration of a replay has parity 1: its shade wrote the parity-0 queues
+            if (h.item_next[0] >= h.item_end && h.ray_count[0] == 0 && h.free_count[0] == 0) break;
+            if (it > max_iters) return hipErrorUnknown;
         }
+        c.timer.begin(c.stream, "wave_finish");
+        hipLaunchKernelGGL(k_finish, dim3((uint32_t)((pixels + 255) / 256)), dim3(256), 0, c.stream, B, films, g.first,
+                           g.second, (uint32_t)pixels);
+        c.timer.end(c.stream);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 hipError_t wave_render(Context& c, uint32_t frame0, uint32_t spp) {
-    std::vector<std::vector<ViewDesc>> layers(spp);
-    for (uint32_t s = 0; s < spp; ++s) {
-        ViewDesc v{};
-        v.cam = c.S.cam;
-        v.film_offset = 0;
-        v.frame = frame0 + s;
-        v.seed = 0;
-        layers[s].push_back(v);
+    ViewDesc v{};
+    v.cam = c.S.cam;
+    v.film_offset = 0;
+    v.frame0 = frame0;
+    v.spp = spp;
+    v.seed = 0;
+    const bool graph = getenv("NOOR_NO_GRAPH") == nullptr;
+    return wave_render_views(c, {v}, c.film, c.wave_capacity, c.wave_block, graph);
+}
+
+hipError_t wave_render_batch(Context& c, const noor_view* v, uint32_t n, float4* films, const size_t* offsets) {
+    std::vector<ViewDesc> views(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        ViewDesc d{};
+        d.cam = v[i].camera;
+        d.film_offset = (uint32_t)offsets[i];
+        d.frame0 = v[i].frame;
+        d.spp = v[i].spp;
+        d.seed = v[i].seed;
+        views[i] = d;
     }
     const bool graph = getenv("NOOR_NO_GRAPH") == nullptr;
-    return wave_render_layers(c, layers, c.film, c.wave_capacity, c.wave_block, graph);
+    return wave_render_views(c, views, films, c.wave_capacity, c.wave_block, graph);
 }
 
 }  // namespace noor

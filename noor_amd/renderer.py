@@ -107,6 +107,43 @@ class Renderer:
     def set_wavefront_params(self, queue_capacity=0, block=64):
         self._check(lib().noor_set_wavefront_params(self._h, queue_capacity, block))
 
+    def set_stack_lds(self, entries=0):
+        self._check(lib().noor_set_stack_lds(self._h, entries))
+
+    @staticmethod
+    def _views(views):
+        """views: list of (camera, frame, spp, seed) tuples or abi.View."""
+        arr = (abi.View * len(views))()
+        for i, v in enumerate(views):
+            if isinstance(v, abi.View):
+                arr[i] = v
+            else:
+                cam, frame, spp, seed = v
+                arr[i] = abi.View(cam, frame, spp, seed, 0)
+        return arr
+
+    def render_batch(self, views):
+        """Renders several views in one wavefront pass (noor_render_batch)."""
+        arr = self._views(views)
+        self._batch = [(arr[i].camera.w, arr[i].camera.h) for i in range(len(arr))]
+        self._check(lib().noor_render_batch(self._h, arr, len(arr)))
+
+    def batch_film(self, i):
+        w, h = self._batch[i]
+        out = np.empty((h, w, 4), np.float32)
+        self._check(lib().noor_read_batch_film(self._h, i, out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
+        return out
+
+    def autotune(self, views):
+        """Times the batch for each wavefront setting and keeps the fastest;
+        returns [(queue_capacity, block, stack_lds, ms)]."""
+        arr = self._views(views)
+        self._batch = [(arr[i].camera.w, arr[i].camera.h) for i in range(len(arr))]
+        out = (abi.TuneEntry * 64)()
+        n = C.c_uint32()
+        self._check(lib().noor_autotune(self._h, arr, len(arr), out, 64, C.byref(n)))
+        return [(out[i].queue_capacity, out[i].block, out[i].stack_lds, out[i].ms) for i in range(n.value)]
+
     def trace(self, rays, any_hit=False):
         n = len(rays)
         hits = (abi.Hit * n)()

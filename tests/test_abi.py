@@ -23,6 +23,7 @@ STRUCTS = {
     "noor_ray": abi.Ray, "noor_hit": abi.Hit, "noor_transform": abi.Transform, "noor_kernel_time": abi.KernelTime,
     "noor_mesh_desc": abi.MeshDesc, "noor_instance_desc": abi.InstanceDesc, "noor_bvh_params": abi.BvhParams,
     "noor_scene_desc": abi.SceneDesc, "noor_build_stats": abi.BuildStats,
+    "noor_view": abi.View, "noor_tune_entry": abi.TuneEntry,
 }
 
 
@@ -160,3 +161,7 @@ def test_context_calls_reject_null_context():
     assert L.noor_trace(None, None, 0, 0, None) == abi.ERR_INVALID_ARG
     assert L.noor_synchronize(None) == abi.ERR_INVALID_ARG
     assert L.noor_destroy(None) == abi.ERR_INVALID_ARG
+    assert L.noor_render_batch(None, None, 0) == abi.ERR_INVALID_ARG
+    assert L.noor_read_batch_film(None, 0, None, 0) == abi.ERR_INVALID_ARG
+    assert L.noor_autotune(None, None, 0, None, 0, None) == abi.ERR_INVALID_ARG
+    assert L.noor_set_stack_lds(None, 0) == abi.ERR_INVALID_ARG
