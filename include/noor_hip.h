@@ -347,6 +347,31 @@ typedef struct noor_tune_entry {
 noor_status noor_autotune(noor_ctx* ctx, const noor_view* views, uint32_t n, noor_tune_entry* out, uint32_t max,
                           uint32_t* count);
 
+/* ---- differentiable rendering (no counterpart in the reference) -------- *
+ * Parameters: the diffuse colour (albedo, RGB) and isotropic roughness of
+ * materials [0, num_materials) and the RGB texels of the HDR environment.
+ * All pointers below are DEVICE pointers on the context's GPU, row-major
+ * float arrays: albedo num_materials x 3, roughness num_materials,
+ * env_rgb env_h x env_w x 3, images h x w x 4 (the fourth channel unused).
+ * Calls are ordered on the context's stream and return after it drains. */
+
+/* Overwrites the parameters (any pointer may be NULL to keep that group);
+ * env_w, env_h must match the environment of the payload.  The environment
+ * sampling distribution is rebuilt from the new texels. */
+noor_status noor_set_params_device(noor_ctx* ctx, const float* albedo, const float* roughness,
+                                   uint32_t num_materials, const float* env_rgb, uint32_t env_w,
+                                   uint32_t env_h);
+/* One sample per pixel from RNG stream `stream` (>= 1), written to
+ * `radiance` instead of the film (the megakernel's estimator). */
+noor_status noor_render_sample(noor_ctx* ctx, uint32_t stream, float* radiance);
+/* Path-replay adjoint of noor_render_sample(stream): given the forward
+ * `radiance` and dLoss/dradiance (`grad_image`), ADDS dLoss/dparameter to
+ * grad_albedo, grad_roughness (num_materials) and grad_env (may be NULL).
+ * Sampling is detached (the random decisions are those of the forward
+ * pass).  Requires MIS off and the per-sample clamp (NOOR_COMPAT_CLAMP) off. */
+noor_status noor_adjoint(noor_ctx* ctx, uint32_t stream, const float* radiance, const float* grad_image,
+                         float* grad_albedo, float* grad_roughness, uint32_t num_materials, float* grad_env);
+
 /* sizeof() of a named ABI struct as compiled into the library (layout checks). */
 size_t noor_abi_struct_size(const char* name);
 

@@ -54,6 +54,11 @@ def lib():
         "noor_batch_film_device_ptr": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p)]),
         "noor_autotune": (C.c_int, [C.c_void_p, P(abi.View), C.c_uint32, P(abi.TuneEntry), C.c_uint32,
                                     P(C.c_uint32)]),
+        "noor_set_params_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                             C.c_uint32]),
+        "noor_render_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+        "noor_adjoint": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                   C.c_void_p]),
         "noor_abi_struct_size": (C.c_size_t, [C.c_char_p]),
         "noor_scene_build": (C.c_int, [P(abi.SceneDesc), P(abi.BvhParams), P(C.c_void_p)]),
         "noor_payload_view_get": (C.c_int, [C.c_void_p, P(abi.PayloadView)]),

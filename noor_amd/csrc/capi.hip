@@ -374,6 +374,7 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
         if ((st = dev_upload(c, (noor_float4**)&S.xf, xf.data(), xf.size())) != NOOR_OK) return bail(st);
     }
     {
+        c->num_materials = P->num_materials;
         std::vector<noor::DevMat> mats(P->num_materials);
         auto rgb = [&](uint32_t t, float* o) {
             const float* px = P->textures[t].rgba;
@@ -685,6 +686,50 @@ noor_status noor_autotune(noor_ctx* ctx, const noor_view* views, uint32_t n, noo
     if (count) *count = (uint32_t)std::min<size_t>(res.size(), max);
     if (out)
         for (uint32_t i = 0; i < std::min<size_t>(res.size(), max); ++i) out[i] = res[i];
+    return NOOR_OK;
+}
+
+noor_status noor_set_params_device(noor_ctx* ctx, const float* albedo, const float* roughness,
+                                   uint32_t num_materials, const float* env_rgb, uint32_t env_w, uint32_t env_h) {
+    if (!ctx) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    if ((albedo || roughness) && num_materials > c->num_materials)
+        return fail(c, NOOR_ERR_INVALID_ARG, "more parameter materials than the payload holds");
+    if (env_rgb) {
+        if (c->S.skydome_type != NOOR_SKY_HDR || !c->S.env)
+            return fail(c, NOOR_ERR_UNSUPPORTED, "environment parameters need an HDR environment");
+        if (env_w != c->S.env_w || env_h != c->S.env_h) return fail(c, NOOR_ERR_INVALID_ARG, "environment size");
+    }
+    HIP_TRY(c, noor::launch_set_params(*c, albedo, roughness, num_materials, env_rgb));
+    if (env_rgb) {
+        noor_status st = build_environment(c);
+        if (st != NOOR_OK) return st;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NOOR_OK;
+}
+
+noor_status noor_render_sample(noor_ctx* ctx, uint32_t stream, float* radiance) {
+    if (!ctx || !radiance || stream < 1) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    HIP_TRY(c, noor::launch_megakernel(*c, stream, 1, (float4*)radiance));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timer.collect();
+    return NOOR_OK;
+}
+
+noor_status noor_adjoint(noor_ctx* ctx, uint32_t stream, const float* radiance, const float* grad_image,
+                         float* grad_albedo, float* grad_roughness, uint32_t num_materials, float* grad_env) {
+    if (!ctx || !radiance || !grad_image || stream < 1) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    if (num_materials && (!grad_albedo || !grad_roughness)) return NOOR_ERR_INVALID_ARG;
+    if (num_materials > c->num_materials) return fail(c, NOOR_ERR_INVALID_ARG, "num_materials");
+    if (c->S.mis) return fail(c, NOOR_ERR_UNSUPPORTED, "the adjoint does not implement MIS");
+    if (c->S.compat & NOOR_COMPAT_CLAMP) return fail(c, NOOR_ERR_UNSUPPORTED, "the adjoint needs the clamp off");
+    HIP_TRY(c, noor::launch_adjoint(*c, stream, radiance, grad_image, grad_albedo, grad_roughness, num_materials,
+                                    grad_env));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timer.collect();
     return NOOR_OK;
 }
 

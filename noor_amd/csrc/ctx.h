@@ -103,6 +103,7 @@ struct Context {
     int stack_lds = 24;           // of which at most this many per lane live in LDS
     size_t wide_nodes = 0;
     uint32_t bvh_root = 0;        // the payload's binary root (the spec must name it)
+    uint32_t num_materials = 0;
     uint32_t* gstack = nullptr;   // device part of the traversal stack
     // multi-view batch films
     float4* batch_film = nullptr;
@@ -123,13 +124,18 @@ inline hipError_t sync_scene(const Context& cc) {
 }
 
 // kernels_mega.hip
-hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp);
+// `direct`: write each pixel's last sample there instead of updating the film
+hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct = nullptr);
 hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits);
 hipError_t launch_debug_skydome(const Context& c, uint32_t frame);
 // kernels_env.hip
 hipError_t launch_sky_bake(const Context& c, float4* out, int w, int h);
 hipError_t launch_dist_build(const Context& c, const float4* tex, int tw, int th, float* func, float* cdf, int W,
                              int H);
+// kernels_adjoint.hip
+hipError_t launch_set_params(const Context& c, const float* albedo, const float* rough, uint32_t n, const float* env_rgb);
+hipError_t launch_adjoint(const Context& c, uint32_t stream, const float* radiance, const float* grad, float* g_albedo,
+                          float* g_rough, uint32_t num_mats, float* g_env);
 // kernels_wave.hip
 hipError_t wave_render(Context& c, uint32_t frame0, uint32_t spp);
 hipError_t wave_render_batch(Context& c, const noor_view* views, uint32_t n, float4* films, const size_t* offsets);

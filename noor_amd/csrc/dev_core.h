@@ -1185,7 +1185,7 @@ __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, const Isect& I, const D
     return matching > 0 ? pdf / (float)matching : 0.0f;
 }
 __device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3& wiW, float& pdf,
-                          int flags, int& sampled, Rng& rng, int rk) {
+                          int flags, int& sampled, Rng& rng, int rk, int* lobe_out = nullptr) {
     int matching = 0;
     for (int i = 0; i < b.n; ++i)
         if (lobe_matches(b.l[i].type, flags)) ++matching;
@@ -1207,7 +1207,8 @@ __device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const D
             li = i;
             break;
         }
-    const Lobe L = b.l[li];
+    const Lobe L = (li == 1) ? b.l[1] : b.l[0];  // a BSDF has at most two lobes
+    if (lobe_out) *lobe_out = li;
     const f3 wo = to_local(I, woW);
     if (wo.z == 0.0f) return f3{0.0f, 0.0f, 0.0f};
     pdf = 0.0f;
@@ -1539,15 +1540,154 @@ __device__ __forceinline__ void resolve_nee(PathState& ps, const Nee& n, bool oc
 }
 
 // Returns true when the path continues with the ray stored in ps.
+// ---------------------------------------------------------------------------------------------
+// parameter derivatives for the adjoint
+
+// d ln(D G) / d alpha of an isotropic GGX lobe (alpha_x = alpha_y), with
+// D = 1 / (pi a^2 cos^4 (1 + e)^2), e = tan^2(wh) / a^2 and
+// Lambda(w) = (sqrt(1 + a^2 tan^2 w) - 1) / 2:
+//   d ln D / da = (2 / a) (e - 1) / (e + 1)
+//   d Lambda / da = a tan^2 / (2 sqrt(1 + a^2 tan^2)),  d ln G / da = -(dLo + dLi) G
+__device__ __forceinline__ float ggx_dlog_dalpha(float a, f3 wo, f3 wi, f3 wh) {
+    float dD = 0.0f;
+    const float t2 = tan2t(wh);
+    if (!is_inf(t2)) {
+        const float e = t2 / (a * a);
+        dD = (2.0f / a) * (e - 1.0f) / (e + 1.0f);
+    }
+    float lam = 0.0f, dlam = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float tt = fabsf(tant(k == 0 ? wo : wi));
+        if (is_inf(tt)) continue;
+        const float q = sqrtf(1.0f + a * a * tt * tt);
+        lam += (q - 1.0f) * 0.5f;
+        dlam += a * tt * tt / (2.0f * q);
+    }
+    return dD - dlam / (1.0f + lam);
+}
+
+// Derivatives of the f that bsdf_eval (lobe < 0, `flags`) or a sample of lobe
+// `lobe` (bsdf_sample, flags B_ALL) sums, with respect to the diffuse colour
+// (dR, per channel: d f_c / d diffuse_c) and the roughness r of an isotropic
+// material (dr, alpha = max(r^2, 1e-3)).
+__device__ __forceinline__ void bsdf_grad(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags,
+                                          int lobe, f3& dR, f3& dr) {
+    dR = f3{0.0f, 0.0f, 0.0f};
+    dr = f3{0.0f, 0.0f, 0.0f};
+    const f3 wo = to_local(I, woW), wi = to_local(I, wiW);
+    const bool reflect = dot3(wiW, I.gn) * dot3(woW, I.gn) > 0.0f;
+    bool spec = false;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        if (i < b.n && i == lobe && (b.l[i].type & B_SPECULAR)) spec = true;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (i >= b.n) break;
+        const Lobe& L = b.l[i];
+        bool inc;
+        if (i == lobe) inc = true;
+        else if (spec) inc = false;
+        else
+            inc = lobe_matches(L.type, flags) &&
+                  ((reflect && (L.type & B_REFLECTION)) || (!reflect && (L.type & B_TRANSMISSION)));
+        if (!inc) continue;
+        switch (L.id) {
+            case L_LAMBERT_R:
+            case L_LAMBERT_T: dR = add3(dR, f3{K_INV_PI, K_INV_PI, K_INV_PI}); break;
+            case L_SPEC_R_NOOP: {
+                if (wi.z == 0.0f) break;
+                const f3 F = fresnel_eval(L.type, m, wi.z);
+                dR = add3(dR, sdiv3(F, fabsf(wi.z)));
+                break;
+            }
+            case L_MICRO_R_DIEL:
+            case L_MICRO_R_COND: {
+                const float r = m.rough[0];
+                if (!(r * r > 1e-3f)) break;
+                const f3 fm = micro_eval(L.type, m, wo, wi);
+                if (is_black(fm)) break;
+                f3 wh = nrm3(add3(wi, wo));
+                wh = scl3(wh, (float)isign(wh.z));
+                const float dl = ggx_dlog_dalpha(r * r, wo, wi, wh);
+                dr = add3(dr, scl3(fm, dl * 2.0f * r));
+                break;
+            }
+            default: break;
+        }
+    }
+}
+
+// Texels and weights of the bilinear environment lookup of sky_eval (HDR).
+struct EnvTaps {
+    uint32_t idx[4];
+    float w[4];
+};
+__device__ __forceinline__ EnvTaps env_taps(const DevScene& S, f3 dir) {
+    EnvTaps t;
+    float u = spherical_phi(dir) * K_INV_2PI, v = spherical_theta(dir) * K_INV_PI;
+    const int w = (int)S.env_w, h = (int)S.env_h;
+    if (u != u) u = 0.0f;
+    if (v != v) v = 0.0f;
+    float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    x = clampr(x, -2.0f, (float)w + 1.0f);
+    y = clampr(y, -2.0f, (float)h + 1.0f);
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int ix = (int)fx0, iy = (int)fy0;
+    const int i0 = ix < 0 ? 0 : (ix > w - 1 ? w - 1 : ix);
+    const int i1 = ix + 1 < 0 ? 0 : (ix + 1 > w - 1 ? w - 1 : ix + 1);
+    const int j0 = iy < 0 ? 0 : (iy > h - 1 ? h - 1 : iy);
+    const int j1 = iy + 1 < 0 ? 0 : (iy + 1 > h - 1 ? h - 1 : iy + 1);
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    t.idx[0] = (uint32_t)(j0 * w + i0);
+    t.w[0] = by * bx;
+    t.idx[1] = (uint32_t)(j0 * w + i1);
+    t.w[1] = by * ax;
+    t.idx[2] = (uint32_t)(j1 * w + i0);
+    t.w[2] = ay * bx;
+    t.idx[3] = (uint32_t)(j1 * w + i1);
+    t.w[3] = ay * ax;
+    return t;
+}
+
+// What the adjoint replay needs from one bounce of shade_bounce(): the sky
+// lookup of an escaping path, the light sample of NEE and the BSDF sample
+// that continued the path, with the hit record they were evaluated at.
+struct Tape {
+    int esc;          // the sky was added for an escaping ray
+    f3 esc_dir, esc_beta;
+    int nee;          // NEE candidate `a` was formed
+    int nee_light;    // its light index
+    f3 nee_wi, nee_Li;
+    float nee_pdf;
+    int scat;         // a BSDF sample continued (or ended) the path
+    int scat_lobe;
+    f3 scat_wi;
+    f3 scat_f;
+    Isect I;
+};
+
 __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, const Ray& ray, bool hit, const Hit& h, Nee& nee,
-                             f3* lookat) {
+                             f3* lookat, Tape* tp = nullptr) {
     const int rk = S.rng_kind;
     nee.has = 0;
     nee.na = 0;
     nee.nb = 0;
+    if (tp) {
+        tp->esc = 0;
+        tp->nee = 0;
+        tp->scat = 0;
+    }
     if (!hit) {
-        if (S.sky_enabled && (ps.bounce == 0 || ps.specular))
+        if (S.sky_enabled && (ps.bounce == 0 || ps.specular)) {
+            if (tp) {
+                tp->esc = 1;
+                tp->esc_dir = nrm3(ray.d);
+                tp->esc_beta = ps.beta;
+            }
             ps.L = add3(ps.L, mul3(ps.beta, sky_eval(S, nrm3(ray.d))));
+        }
         return false;
     }
     Isect I;
@@ -1589,6 +1729,14 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
                 nee.a = c;
                 nee.ra = shadow_ray_vis(S, I, vis);
                 nee.na = 1;
+                if (tp) {
+                    tp->I = I;
+                    tp->nee = 1;
+                    tp->nee_light = light_idx;
+                    tp->nee_wi = vis.wi;
+                    tp->nee_Li = Li;
+                    tp->nee_pdf = light_pdf;
+                }
             }
         }
     }
@@ -1631,9 +1779,17 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
     }
     float pdf = 0.0f;
     int sampled = 0;
+    int lobe = -1;
     f3 wi = I.wi;
-    const f3 f = bsdf_sample(b, I, m, I.wo, wi, pdf, B_ALL, sampled, ps.rng, rk);
+    const f3 f = bsdf_sample(b, I, m, I.wo, wi, pdf, B_ALL, sampled, ps.rng, rk, &lobe);
     I.wi = wi;
+    if (tp) {
+        tp->scat = (pdf != 0.0f) ? 1 : 0;
+        tp->scat_lobe = lobe;
+        tp->scat_wi = wi;
+        tp->scat_f = f;
+        tp->I = I;
+    }
     const f3 sc = (pdf == 0.0f) ? f3{0.0f, 0.0f, 0.0f} : sdiv3(scl3(f, absdot3(I.wi, I.sn)), pdf);
     ps.beta = mul3(ps.beta, sc);
     // correct_sidedness (noor/cuda/accumulate.cuh:41-45)

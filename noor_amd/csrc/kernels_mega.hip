@@ -40,7 +40,7 @@ __device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext
 template <bool STATS>
 __global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ Sp, float4* __restrict__ film, uint32_t frame0,
                                                    uint32_t spp, DevStats* __restrict__ stats,
-                                                   float4* __restrict__ lookat_out) {
+                                                   float4* __restrict__ lookat_out, float4* __restrict__ direct) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x & 63;
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ 
         const uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
         if (x >= W || y >= H) continue;
         const uint32_t gid = y * W + x;
-        float4 acc = film[gid];
+        float4 acc = direct ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : film[gid];
         f3 lookat = f3{0.0f, 0.0f, 0.0f};
         const bool center = (gid == S.cam.center);
         for (uint32_t s = 0; s < spp; ++s) {
@@ -87,15 +87,21 @@ __global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ 
                 ray = mkray(ps.o, ps.d, K_INF);
                 ps.bounce++;
             }
-            acc = film_lerp(acc, final_radiance(S, ps.L), frame);
+            if (direct) {
+                const f3 c = final_radiance(S, ps.L);
+                acc = make_float4(c.x, c.y, c.z, 1.0f);
+            } else {
+                acc = film_lerp(acc, final_radiance(S, ps.L), frame);
+            }
         }
-        film[gid] = acc;
+        if (direct) direct[gid] = acc;
+        else film[gid] = acc;
         if (center && lookat_out) *lookat_out = make_float4(lookat.x, lookat.y, lookat.z, 1.0f);
     }
     flush_stats(stats, ext, shd, cnt, cns, overflow);
 }
 
-hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp) {
+hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct) {
     hipError_t se = sync_scene(c);
     if (se != hipSuccess) return se;
     const uint32_t tiles = ((c.S.cam.w + 7) / 8) * ((c.S.cam.h + 7) / 8);
@@ -106,10 +112,10 @@ hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp) {
     T.begin(c.stream, "megakernel");
     if (c.opt.stats)
         hipLaunchKernelGGL(k_megakernel<true>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
-                           c.d_stats, c.d_lookat);
+                           c.d_stats, c.d_lookat, direct);
     else
         hipLaunchKernelGGL(k_megakernel<false>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
-                           c.d_stats, c.d_lookat);
+                           c.d_stats, c.d_lookat, direct);
     T.end(c.stream);
     return hipGetLastError();
 }

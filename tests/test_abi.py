@@ -165,3 +165,6 @@ def test_context_calls_reject_null_context():
     assert L.noor_read_batch_film(None, 0, None, 0) == abi.ERR_INVALID_ARG
     assert L.noor_autotune(None, None, 0, None, 0, None) == abi.ERR_INVALID_ARG
     assert L.noor_set_stack_lds(None, 0) == abi.ERR_INVALID_ARG
+    assert L.noor_set_params_device(None, None, None, 0, None, 0, 0) == abi.ERR_INVALID_ARG
+    assert L.noor_render_sample(None, 1, None) == abi.ERR_INVALID_ARG
+    assert L.noor_adjoint(None, 1, None, None, None, None, 0, None) == abi.ERR_INVALID_ARG

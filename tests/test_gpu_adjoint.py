@@ -1,0 +1,153 @@
+"""GPU: the path-replay adjoint against FP64 central differences.
+
+The loss is L2 between a one-sample render at the start parameters (all
+0.5) and the one-sample render at the seeded ground truth.  Finite
+differences use the FP64 oracle with detached sampling: the random decisions
+come from a frozen copy of the start parameters and only the evaluation
+parameters move, so the FD and the adjoint differentiate the same smooth
+per-seed function.  Tolerance: 1e-3 relative, against max(|FD|, 1% of the
+largest |FD| of the parameter group)."""
+import numpy as np
+import pytest
+import torch
+
+from noor_amd import abi, scenes
+from noor_amd import autograd as nag
+from noor_amd.renderer import Renderer
+from oracle.binding import OracleScene, render_detached_f64
+
+pytestmark = pytest.mark.gpu
+
+H_FD = 1e-3
+TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def setup():
+    wl = scenes.adjoint_scene(subdiv=3, width=48, height=48)
+    M = wl.info["n_materials"]
+    env_gt = wl.info["env"]
+    eh, ew = env_gt.shape[:2]
+    dev = torch.device("cuda:0")
+    r = Renderer.from_workload(wl)
+    gt = dict(albedo=torch.tensor(wl.info["albedo"], dtype=torch.float32, device=dev),
+              rough=torch.tensor(wl.info["roughness"], dtype=torch.float32, device=dev),
+              env=torch.tensor(env_gt[..., :3], dtype=torch.float32, device=dev))
+    with torch.no_grad():
+        target = nag.render(r, 1, gt["albedo"], gt["rough"], gt["env"])
+    p0 = dict(albedo=torch.full((M, 3), 0.5, device=dev), rough=torch.full((M,), 0.5, device=dev),
+              env=torch.full((eh, ew, 3), 0.5, device=dev))
+    return wl, r, target, p0
+
+
+def _oracle_scenes(wl, p0):
+    M = wl.info["n_materials"]
+    env = np.ones((*p0["env"].shape[:2], 4), np.float32)
+    env[..., :3] = p0["env"].cpu().numpy()
+    out = []
+    for _ in range(2):
+        o = OracleScene.from_workload(wl)
+        for m in range(M):
+            o.set_material(m, diffuse=p0["albedo"][m].tolist(), rough=(float(p0["rough"][m]),) * 2)
+        o.set_env(env)
+        out.append(o)
+    return out, env
+
+
+def test_adjoint_matches_fp64_central_differences(setup):
+    wl, r, target, p0 = setup
+    a = p0["albedo"].clone().requires_grad_(True)
+    ro = p0["rough"].clone().requires_grad_(True)
+    e = p0["env"].clone().requires_grad_(True)
+    img = nag.render(r, 1, a, ro, e)
+    loss = ((img - target) ** 2).sum()
+    loss.backward()
+    ga, gr, ge = a.grad.cpu().numpy(), ro.grad.cpu().numpy(), e.grad.cpu().numpy()
+    T = target.cpu().numpy().astype(np.float64)
+    (o_eval, o_frozen), env0 = _oracle_scenes(wl, p0)
+
+    def loss64():
+        f = render_detached_f64(o_eval, o_frozen, 1, 1)
+        return float(((f[..., :3] - T) ** 2).sum())
+
+    base = loss64()
+    print(f"loss gpu {float(loss):.6f} oracle fp64 {base:.6f}")
+    assert base == pytest.approx(float(loss), rel=1e-4)
+    # parameters: the largest gradients of each group plus a few others
+    rng = np.random.default_rng(0)
+    M = ga.shape[0]
+    picks = []
+    for m in list(np.argsort(-np.abs(gr))[:6]) + list(rng.choice(M, 2, replace=False)):
+        picks.append(("rough", int(m), 0))
+    flat = np.abs(ga).ravel()
+    for k in list(np.argsort(-flat)[:6]) + list(rng.choice(flat.size, 2, replace=False)):
+        picks.append(("albedo", int(k // 3), int(k % 3)))
+    flat = np.abs(ge).ravel()
+    for k in np.argsort(-flat)[:10]:
+        picks.append(("env", int(k // 3), int(k % 3)))
+    results = {"rough": [], "albedo": [], "env": []}
+    for kind, i, c in picks:
+        vals = []
+        for s in (+1, -1):
+            if kind == "rough":
+                v = float(p0["rough"][i]) + s * H_FD
+                o_eval.set_material(i, rough=(v, v))
+            elif kind == "albedo":
+                d = p0["albedo"][i].tolist()
+                d[c] += s * H_FD
+                o_eval.set_material(i, diffuse=d)
+            else:
+                env = env0.copy()
+                env.reshape(-1, 4)[i, c] += s * H_FD
+                o_eval.set_env(env)
+            vals.append(loss64())
+        # restore
+        if kind == "rough":
+            o_eval.set_material(i, rough=(float(p0["rough"][i]),) * 2)
+        elif kind == "albedo":
+            o_eval.set_material(i, diffuse=p0["albedo"][i].tolist())
+        else:
+            o_eval.set_env(env0)
+        fd = (vals[0] - vals[1]) / (2 * H_FD)
+        g = {"rough": gr[i], "albedo": ga[i, c], "env": ge.reshape(-1, 3)[i, c]}[kind]
+        results[kind].append((float(g), fd))
+        print(f"{kind:6s} {i:6d}.{c}  adjoint {g: .6e}  fd {fd: .6e}", flush=True)
+    for kind, rows in results.items():
+        g = np.array([x for x, _ in rows])
+        fd = np.array([y for _, y in rows])
+        floor = 1e-2 * np.abs(fd).max()
+        rel = np.abs(g - fd) / np.maximum(np.abs(fd), floor)
+        print(f"{kind}: max rel err {rel.max():.2e}")
+        assert rel.max() <= TOL, (kind, rel)
+
+
+def test_autograd_optimisation_descends(setup):
+    wl, r, target, p0 = setup
+    params = [p0["albedo"].clone().requires_grad_(True), p0["rough"].clone().requires_grad_(True),
+              p0["env"].clone().requires_grad_(True)]
+    opt = torch.optim.Adam(params, lr=0.02)
+    losses = []
+    for it in range(12):
+        opt.zero_grad()
+        img = nag.render(r, 1 + it, *params)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        assert all(torch.isfinite(p.grad).all() for p in params)
+        opt.step()
+        with torch.no_grad():
+            params[0].clamp_(0.0, 1.0)
+            params[1].clamp_(0.05, 1.0)
+            params[2].clamp_(0.0, None)
+        losses.append(float(loss))
+    print("losses", [f"{v:.4f}" for v in losses])
+    assert losses[-1] < losses[0]
+
+
+def test_adjoint_rejects_clamp_and_mis(setup):
+    wl, r, target, p0 = setup
+    opt = abi.Options(wl.options.rng, wl.options.tri_test, abi.COMPAT_CLAMP, 0)
+    r2 = Renderer(wl.payload, wl.camera, wl.spec, opt, wl.hosek)
+    a = p0["albedo"].clone().requires_grad_(True)
+    img = nag.render(r2, 1, a, p0["rough"], None)
+    with pytest.raises(abi.NoorError):
+        img.sum().backward()
