@@ -17,8 +17,14 @@ plus its per-ray queue and record traffic) divided by its average launch duratio
 the renderer's own stream during an extra, event-timed step.
 
 `cpu_baseline`: the CPU restatement of the same path (the oracle, FP32) on
-all host cores over a bounded sample of the same frame (1 spp over a band of
-rows), in Mrays/s.
+all host cores over a bounded sample of the same frame (a band of rows, or
+the whole frame at several spp, about 15 s of CPU work), in Mrays/s; the
+sample's pixels are compared with the GPU's film of the same rows and frames.
+
+`batch`: 48 views of the 1M-triangle Hosek-sky scene rendered as one
+wavefront batch (autotuned) against the megakernel view by view.
+`adjoint`: forward and path-replay backward times of the 512^2 one-sample
+differentiable render (bwd/fwd ratio).
 """
 import argparse
 import json
@@ -64,7 +70,7 @@ def cpu_baseline(wl, target_s):
         want_rows, spp = H, max(1, min(wl.spp, int(want / H)))
     y0 = (H - want_rows) // 2
     t = time.time()
-    _, cnt = o.render(1, spp, threads=cores, rect=(0, y0, W, y0 + want_rows))
+    film, cnt = o.render(1, spp, threads=cores, rect=(0, y0, W, y0 + want_rows))
     dt = time.time() - t
     rays = cnt.extension_rays + cnt.shadow_rays
     log(f"cpu baseline: {want_rows} rows x {W} px x {spp} spp, {rays} rays in {dt:.1f}s on {cores} threads")
@@ -76,7 +82,102 @@ def cpu_baseline(wl, target_s):
         "sample": f"rows {y0}-{y0 + want_rows} of the {W}x{H} frame at {spp} spp (frames 1-{spp}), "
                   f"FP32 oracle, {rays} rays in {dt:.1f} s",
         "spp_per_s": round(want_rows * W * spp / dt, 1),
-    }
+    }, film, (y0, y0 + want_rows), spp
+
+
+def bench_batch(log_prefix="batch"):
+    """48 views of the 1M-triangle procedural scene (Hosek sky), resolution in
+    {256^2, 512^2, 1280x720, 1920x1080} and spp in {1, 4, 16, 64}: one
+    wavefront batch (after autotuning on a one-sample copy) against rendering
+    the views one by one with the megakernel."""
+    from noor_amd import scenes
+    from noor_amd.renderer import Renderer
+    wl = scenes.procedural(width=256, height=256)
+    views = scenes.batch_views(48)
+    samples = sum(c.w * c.h * spp for c, _, spp, _ in views)
+    r = Renderer.from_workload(wl)
+    t = time.perf_counter()
+    tune = r.autotune([(c, 1, 1, sd) for c, _, _, sd in views])
+    t_tune = time.perf_counter() - t
+    best = min(tune, key=lambda x: x[3])
+    log(f"{log_prefix}: autotune {len(tune)} settings in {t_tune:.1f}s, best capacity {best[0]} block {best[1]} "
+        f"lds {best[2]} ({best[3]:.1f} ms for the one-sample batch)")
+    # rays of the batch (counting pass, untimed)
+    wl.options.stats = 1
+    r.update_options(wl.options)
+    r.reset_stats()
+    r.render_batch(views)
+    st = r.stats()
+    wl.options.stats = 0
+    r.update_options(wl.options)
+    rays = st["extension_rays"] + st["shadow_rays"]
+    r.render_batch(views)  # warm-up
+    r.synchronize()
+    t = time.perf_counter()
+    r.render_batch(views)
+    r.synchronize()
+    t_batch = time.perf_counter() - t
+    log(f"{log_prefix}: {samples} samples, {rays} rays in {t_batch:.3f}s (wavefront batch)")
+    t_mega = 0.0
+    for c, f, spp, _ in views:
+        r.update_camera(c)
+        r.render(spp=1, frame=1)  # allocation and warm-up of this size
+        r.synchronize()
+        t = time.perf_counter()
+        r.render(spp=spp, frame=1)
+        r.synchronize()
+        t_mega += time.perf_counter() - t
+    log(f"{log_prefix}: megakernel view by view {t_mega:.3f}s")
+    r.close()
+    return {"views": len(views), "samples": samples, "rays": rays,
+            "wavefront_batch_s": round(t_batch, 4), "megakernel_per_view_s": round(t_mega, 4),
+            "wavefront_mrays_per_s": round(rays / t_batch / 1e6, 2),
+            "megakernel_mrays_per_s": round(rays / t_mega / 1e6, 2),
+            "batch_vs_per_view": round(t_mega / t_batch, 3),
+            "autotune": {"settings": len(tune), "seconds": round(t_tune, 1),
+                         "best": {"queue_capacity": best[0], "block": best[1], "stack_lds": best[2]}}}
+
+
+def bench_adjoint(passes=10):
+    """Forward (one-sample render) and path-replay backward on the 512^2,
+    depth-4 Cornell scene with 64 glossy materials and a 512x256 environment."""
+    import ctypes as C
+    import torch
+    from noor_amd import scenes
+    from noor_amd._lib import lib
+    from noor_amd.renderer import Renderer
+    wl = scenes.adjoint_scene()
+    M = wl.info["n_materials"]
+    eh, ew = wl.info["env"].shape[:2]
+    dev = torch.device("cuda:0")
+    r = Renderer.from_workload(wl)
+    alb = torch.full((M, 3), 0.5, device=dev)
+    rough = torch.full((M,), 0.5, device=dev)
+    env = torch.full((eh, ew, 3), 0.5, device=dev)
+    img = torch.empty((wl.camera.h, wl.camera.w, 4), device=dev)
+    grad = torch.ones_like(img) * 1e-3
+    ga, gr, ge = torch.zeros_like(alb), torch.zeros_like(rough), torch.zeros_like(env)
+    torch.cuda.synchronize()
+    P = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    L = lib()
+    assert L.noor_set_params_device(r._h, P(alb), P(rough), M, P(env), ew, eh) == 0
+    for s in range(1, 3):
+        assert L.noor_render_sample(r._h, s, P(img)) == 0
+        assert L.noor_adjoint(r._h, s, P(img), P(grad), P(ga), P(gr), M, P(ge)) == 0
+    r.set_timing(True)
+    r.reset_kernel_times()
+    for s in range(3, 3 + passes):
+        assert L.noor_render_sample(r._h, s, P(img)) == 0
+        assert L.noor_adjoint(r._h, s, P(img), P(grad), P(ga), P(gr), M, P(ge)) == 0
+    kt = r.kernel_times()
+    r.set_timing(False)
+    fwd = kt["megakernel"][1] / kt["megakernel"][0]
+    bwd = kt["adjoint"][1] / kt["adjoint"][0]
+    log(f"adjoint: forward {fwd:.3f} ms, backward {bwd:.3f} ms per one-sample pass ({passes} passes)")
+    r.close()
+    return {"workload": f"adjoint cornell {wl.payload.view.num_triangles} tris, {wl.camera.w}x{wl.camera.h}, "
+                        f"1 spp, {wl.spec.bounces} bounces, {M} materials, env {ew}x{eh}",
+            "fwd_ms": round(fwd, 4), "bwd_ms": round(bwd, 4), "bwd_over_fwd": round(bwd / fwd, 3)}
 
 
 def main():
@@ -93,6 +194,8 @@ def main():
     ap.add_argument("--skip-mega", action="store_true")
     ap.add_argument("--queue", type=int, default=0)
     ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -219,10 +322,26 @@ def main():
         out["image_mean_rel_err_wave_vs_mega"] = float(np.mean(diff / (np.abs(mega_film[..., :3]) + 1e-3)))
     if not args.no_cpu:
         wl.options.stats = 0
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+        cb, cpu_film, rows, cpu_spp = cpu_baseline(wl, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        # the CPU sample against the GPU film of the same rows and samples
+        if cpu_spp != spp:
+            r.render(spp=cpu_spp, frame=1)
+            ref = r.film()
+        else:
+            ref = wave_film
+        g = ref[rows[0]:rows[1]]
+        out["cpu_vs_gpu_identical_pixels"] = float(np.mean(np.all(g == cpu_film, axis=-1)))
+        out["cpu_vs_gpu_mean_rel_err"] = float(np.mean(np.abs(g[..., :3].astype(np.float64) - cpu_film[..., :3]) /
+                                                       (np.abs(cpu_film[..., :3]) + 1e-3)))
     else:
         out["cpu_baseline"] = None
     r.close()
+    del r
+    if not args.no_batch:
+        out["batch"] = bench_batch()
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint()
     print(json.dumps(out), flush=True)
 
 

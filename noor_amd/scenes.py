@@ -400,3 +400,24 @@ def adjoint_scene(seed=4, subdiv=6, width=512, height=512, bounces=4, rr=3, n_ma
     opt = make_options(rng=rng_kind, compat=0)
     return Workload("adjoint", payload, cam, spec, opt, None, 1,
                     info=dict(albedo=albedo, roughness=rough, env=env, n_materials=M))
+
+
+def batch_views(n=48, seed=7, center=(50.0, 45.0, 50.0), sizes=((256, 256), (512, 512), (1280, 720), (1920, 1080)),
+                spps=(1, 4, 16, 64)):
+    """`n` seeded views of the procedural scene for batched rendering: eye on a
+    shell around the scene, resolution and spp drawn uniformly from `sizes`
+    and `spps`, a per-view RNG seed.  Returns [(camera, frame, spp, seed)]."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(center, np.float64)
+    out = []
+    for i in range(n):
+        d = rng.normal(size=3)
+        d[1] = abs(d[1]) + 0.3
+        d /= np.linalg.norm(d)
+        eye = c + d * rng.uniform(120.0, 220.0)
+        look = c + rng.normal(size=3) * 10.0
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        spp = int(spps[int(rng.integers(len(spps)))])
+        cam = make_camera(tuple(eye), tuple(look), (0, 1, 0), np.radians(rng.uniform(35.0, 60.0)), w, h)
+        out.append((cam, 1, spp, int(rng.integers(1 << 20))))
+    return out
