@@ -372,6 +372,13 @@ noor_status noor_render_sample(noor_ctx* ctx, uint32_t stream, float* radiance);
 noor_status noor_adjoint(noor_ctx* ctx, uint32_t stream, const float* radiance, const float* grad_image,
                          float* grad_albedo, float* grad_roughness, uint32_t num_materials, float* grad_env);
 
+/* Host-side check of the four-wide BVH the traversal kernels use (no GPU
+ * needed): builds it from a payload and verifies that every triangle of every
+ * instanced BLAS is reachable exactly once per BLAS, that every child box
+ * equals the box of the binary node it came from, and the stack bound.
+ * Reports the node count and the stack entries a traversal can need. */
+noor_status noor_wide_bvh_check(const noor_payload_view* payload, uint32_t* num_nodes, uint32_t* stack_need);
+
 /* sizeof() of a named ABI struct as compiled into the library (layout checks). */
 size_t noor_abi_struct_size(const char* name);
 

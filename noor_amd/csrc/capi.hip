@@ -733,6 +733,23 @@ noor_status noor_adjoint(noor_ctx* ctx, uint32_t stream, const float* radiance, 
     return NOOR_OK;
 }
 
+noor_status noor_wide_bvh_check(const noor_payload_view* P, uint32_t* num_nodes, uint32_t* stack_need_out) {
+    if (!P) return NOOR_ERR_INVALID_ARG;
+    Context tmp;
+    noor_status st = validate(&tmp, P);
+    if (st != NOOR_OK) return st;
+    std::vector<int32_t> memo(P->num_bvh_nodes, -1);
+    bool bad = false;
+    stack_need(P, P->bvh_root_node, memo, 0, &bad);
+    if (bad) return NOOR_ERR_INVALID_PAYLOAD;
+    noor::WideBvh wb;
+    if (!noor::build_wide_bvh(P, wb).empty()) return NOOR_ERR_UNSUPPORTED;
+    if (!noor::check_wide_bvh(P, wb).empty()) return NOOR_ERR_INVALID_PAYLOAD;
+    if (num_nodes) *num_nodes = (uint32_t)wb.nodes.size();
+    if (stack_need_out) *stack_need_out = wb.stack_need;
+    return NOOR_OK;
+}
+
 noor_status noor_debug_skydome(noor_ctx* ctx, uint32_t* frame) {
     if (!ctx || !frame) return NOOR_ERR_INVALID_ARG;
     Context* c = &ctx->c;

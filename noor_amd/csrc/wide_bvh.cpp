@@ -274,6 +274,103 @@ class Builder {
 
 }  // namespace
 
+namespace {
+
+bool inside(const float lo[3], const float hi[3], const WNode& n, int c) {
+    for (int k = 0; k < 3; ++k)
+        if (lo[k] < n.lo[k][c] || hi[k] > n.hi[k][c]) return false;
+    return true;
+}
+
+// Triangles under binary node b of one BLAS.
+void binary_tris(const noor_payload_view* P, uint32_t b, std::vector<uint32_t>& out) {
+    std::vector<uint32_t> st{b};
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        const uint32_t w = P->bvh_max_axis_count[i].w;
+        if (w & (NOOR_LIGHT_NODE_MASK | NOOR_MESH_NODE_MASK)) continue;
+        if (w & NOOR_TRI_LEAF_MASK) {
+            const uint32_t s = P->bvh_min_right_start[i].w, n = w & NOOR_TRI_COUNT_MASK;
+            for (uint32_t t = s; t < s + n; ++t) out.push_back(t);
+            continue;
+        }
+        st.push_back(i + 1);
+        st.push_back(P->bvh_min_right_start[i].w);
+    }
+}
+
+}  // namespace
+
+std::string check_wide_bvh(const noor_payload_view* P, const WideBvh& W) {
+    const size_t nn = W.nodes.size();
+    // BLAS roots: one per transform, plus the root when it is not a TLAS node
+    std::vector<std::pair<uint32_t, uint32_t>> roots;  // (wide ref, binary root)
+    for (uint32_t i = 0; i < P->num_bvh_nodes; ++i) {
+        const uint32_t w = P->bvh_max_axis_count[i].w;
+        if ((w & NOOR_LIGHT_NODE_MASK) || !(w & NOOR_MESH_INSTANCE_MASK)) continue;
+        const uint32_t ins = w & NOOR_TRANS_MASK;
+        if (ins >= W.inst_root.size()) return "instance out of range";
+        roots.push_back({W.inst_root[ins], P->bvh_min_right_start[i].w});
+    }
+    if (!(P->bvh_max_axis_count[P->bvh_root_node].w & (NOOR_MESH_NODE_MASK | NOOR_LIGHT_NODE_MASK)))
+        roots.push_back({W.root, P->bvh_root_node});
+    std::sort(roots.begin(), roots.end());
+    roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+    for (const auto& r : roots) {
+        std::vector<uint32_t> want;
+        binary_tris(P, r.second, want);
+        std::vector<uint32_t> got;
+        // walk the wide BLAS: every child inside its parent's entry, leaves around their triangles
+        std::vector<uint32_t> st{r.first};
+        while (!st.empty()) {
+            const uint32_t ref = st.back();
+            st.pop_back();
+            const uint32_t kind = (ref >> REF_KIND_SHIFT) & 3u;
+            if (ref == REF_EMPTY) continue;
+            if (kind == REF_LEAF) {
+                const uint32_t s = ref & REF_LEAF_START_MASK, n = ((ref >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
+                for (uint32_t t = s; t < s + n; ++t) got.push_back(t);
+                continue;
+            }
+            if (kind != REF_INNER) return "a BLAS references a TLAS node";
+            const uint32_t idx = ref & REF_INDEX_MASK;
+            if (idx >= nn) return "wide node index out of range";
+            const WNode& n = W.nodes[idx];
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t cr = n.ref[c];
+                if (cr == REF_EMPTY) continue;
+                const uint32_t ck = (cr >> REF_KIND_SHIFT) & 3u;
+                if (ck == REF_INNER) {
+                    const WNode& m = W.nodes[cr & REF_INDEX_MASK];
+                    for (int d = 0; d < 4; ++d) {
+                        if (m.ref[d] == REF_EMPTY) continue;
+                        const float lo[3] = {m.lo[0][d], m.lo[1][d], m.lo[2][d]};
+                        const float hi[3] = {m.hi[0][d], m.hi[1][d], m.hi[2][d]};
+                        if (!inside(lo, hi, n, c)) return "child box outside its parent's";
+                    }
+                } else if (ck == REF_LEAF) {
+                    const uint32_t s = cr & REF_LEAF_START_MASK, cnt = ((cr >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
+                    for (uint32_t t = s; t < s + cnt; ++t) {
+                        const noor_uint4 a = P->attribute_indices[t];
+                        for (uint32_t v : {a.x, a.y, a.z}) {
+                            const noor_float4 p = P->vertices[v];
+                            const float q[3] = {p.x, p.y, p.z};
+                            if (!inside(q, q, n, c)) return "triangle outside its leaf box";
+                        }
+                    }
+                }
+                st.push_back(cr);
+            }
+        }
+        std::sort(want.begin(), want.end());
+        std::sort(got.begin(), got.end());
+        if (want != got) return "a BLAS does not cover its triangles exactly once";
+    }
+    if (W.stack_need < 1) return "stack bound";
+    return std::string();
+}
+
 std::string build_wide_bvh(const noor_payload_view* P, WideBvh& out) {
     Builder b(P, out);
     return b.run();

@@ -54,12 +54,15 @@ struct WideBvh {
     std::vector<uint32_t> inst_root;  // per transform: reference of its BLAS root
     uint32_t root = REF_EMPTY;
     uint32_t stack_need = 0;          // most stack entries any traversal can hold
-    uint64_t binary_inner = 0;        // binary nodes folded into wide nodes (statistics)
 };
 
 // Builds the wide BVH of a validated payload.  Returns an empty string on
 // success, else the reason the payload cannot be represented.
 std::string build_wide_bvh(const noor_payload_view* P, WideBvh& out);
+
+// Structural check of a built wide BVH against the payload (see
+// noor_wide_bvh_check).  Returns an empty string when it holds.
+std::string check_wide_bvh(const noor_payload_view* P, const WideBvh& W);
 
 }  // namespace noor
 

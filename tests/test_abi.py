@@ -168,3 +168,33 @@ def test_context_calls_reject_null_context():
     assert L.noor_set_params_device(None, None, None, 0, None, 0, 0) == abi.ERR_INVALID_ARG
     assert L.noor_render_sample(None, 1, None) == abi.ERR_INVALID_ARG
     assert L.noor_adjoint(None, 1, None, None, None, None, 0, None) == abi.ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("name", ["cornell", "procedural", "terrain"])
+def test_wide_bvh_covers_every_triangle_once(name):
+    """The four-wide BVH of the traversal kernels, built on the host from the
+    payload's binary BVH: exact triangle coverage per BLAS, nested boxes,
+    leaf boxes around their triangles."""
+    kw = dict(width=16, height=16)
+    if name == "terrain":
+        kw.update(grid=200, n_spheres=50, env_w=64, env_h=32)
+    if name == "procedural":
+        kw.update(target_tris=80000, n_meshes=40)
+    wl = getattr(scenes, name)(**kw)
+    n, need = C.c_uint32(), C.c_uint32()
+    assert lib().noor_wide_bvh_check(C.byref(wl.payload.view), C.byref(n), C.byref(need)) == abi.OK
+    binary = wl.payload.view.num_bvh_nodes
+    assert 0 < n.value < binary  # four-wide: far fewer nodes than the binary tree
+    assert 1 <= need.value <= 3 * 64
+
+
+def test_wide_bvh_check_rejects_corrupt_payload(small):
+    arr = small.payload.arrays()
+    v = _copy_view(small)
+    a = arr["bvh_a"].copy()
+    b = arr["bvh_b"]
+    inner = [i for i in range(len(a)) if (int(b[i, 3]) & 0xF8000000) == 0][0]
+    a[inner, 3] = v.num_bvh_nodes + 5
+    keep = np.ascontiguousarray(a, np.uint32)
+    v.bvh_min_right_start = keep.ctypes.data_as(C.POINTER(abi.uint4))
+    assert lib().noor_wide_bvh_check(C.byref(v), None, None) == abi.ERR_INVALID_PAYLOAD
