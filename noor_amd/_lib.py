@@ -20,6 +20,12 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    # torch bundles its own HIP runtime under the same soname as the system's:
+    # loading it first makes both use one runtime in this process
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P = C.POINTER
     sigs = {

@@ -1781,7 +1781,7 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
     int sampled = 0;
     int lobe = -1;
     f3 wi = I.wi;
-    const f3 f = bsdf_sample(b, I, m, I.wo, wi, pdf, B_ALL, sampled, ps.rng, rk, &lobe);
+    const f3 f = bsdf_sample(b, I, m, I.wo, wi, pdf, B_ALL, sampled, ps.rng, rk, tp ? &lobe : nullptr);
     I.wi = wi;
     if (tp) {
         tp->scat = (pdf != 0.0f) ? 1 : 0;

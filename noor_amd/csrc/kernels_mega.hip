@@ -37,7 +37,9 @@ __device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext
     flush_counters(st->shd, cs);
 }
 
-template <bool STATS>
+// DIRECT: write each pixel's last sample to `direct` instead of updating the
+// film (a separate instantiation keeps the film path's register budget).
+template <bool STATS, bool DIRECT>
 __global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ Sp, float4* __restrict__ film, uint32_t frame0,
                                                    uint32_t spp, DevStats* __restrict__ stats,
                                                    float4* __restrict__ lookat_out, float4* __restrict__ direct) {
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ 
         const uint32_t y = (tile / tiles_x) * 8 + (lane >> 3);
         if (x >= W || y >= H) continue;
         const uint32_t gid = y * W + x;
-        float4 acc = direct ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : film[gid];
+        float4 acc = DIRECT ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : film[gid];
         f3 lookat = f3{0.0f, 0.0f, 0.0f};
         const bool center = (gid == S.cam.center);
         for (uint32_t s = 0; s < spp; ++s) {
@@ -87,14 +89,14 @@ __global__ __launch_bounds__(64) void k_megakernel(const DevScene* __restrict__ 
                 ray = mkray(ps.o, ps.d, K_INF);
                 ps.bounce++;
             }
-            if (direct) {
+            if (DIRECT) {
                 const f3 c = final_radiance(S, ps.L);
                 acc = make_float4(c.x, c.y, c.z, 1.0f);
             } else {
                 acc = film_lerp(acc, final_radiance(S, ps.L), frame);
             }
         }
-        if (direct) direct[gid] = acc;
+        if (DIRECT) direct[gid] = acc;
         else film[gid] = acc;
         if (center && lookat_out) *lookat_out = make_float4(lookat.x, lookat.y, lookat.z, 1.0f);
     }
@@ -110,12 +112,15 @@ hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, fl
     const size_t lds = (size_t)64 * c.S.stack_lds * sizeof(uint32_t);
     KernelTimer& T = const_cast<KernelTimer&>(c.timer);
     T.begin(c.stream, "megakernel");
-    if (c.opt.stats)
-        hipLaunchKernelGGL(k_megakernel<true>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
-                           c.d_stats, c.d_lookat, direct);
+    if (direct)
+        hipLaunchKernelGGL((k_megakernel<false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
+                           spp, c.d_stats, c.d_lookat, direct);
+    else if (c.opt.stats)
+        hipLaunchKernelGGL((k_megakernel<true, false>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
+                           spp, c.d_stats, c.d_lookat, direct);
     else
-        hipLaunchKernelGGL(k_megakernel<false>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0, spp,
-                           c.d_stats, c.d_lookat, direct);
+        hipLaunchKernelGGL((k_megakernel<false, false>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
+                           spp, c.d_stats, c.d_lookat, direct);
     T.end(c.stream);
     return hipGetLastError();
 }

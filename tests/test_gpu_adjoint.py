@@ -71,8 +71,8 @@ def test_adjoint_matches_fp64_central_differences(setup):
         return float(((f[..., :3] - T) ** 2).sum())
 
     base = loss64()
-    print(f"loss gpu {float(loss):.6f} oracle fp64 {base:.6f}")
-    assert base == pytest.approx(float(loss), rel=1e-4)
+    print(f"loss gpu {loss.item():.6f} oracle fp64 {base:.6f}")
+    assert base == pytest.approx(loss.item(), rel=1e-4)
     # parameters: the largest gradients of each group plus a few others
     rng = np.random.default_rng(0)
     M = ga.shape[0]
@@ -109,7 +109,12 @@ def test_adjoint_matches_fp64_central_differences(setup):
         else:
             o_eval.set_env(env0)
         fd = (vals[0] - vals[1]) / (2 * H_FD)
-        g = {"rough": gr[i], "albedo": ga[i, c], "env": ge.reshape(-1, 3)[i, c]}[kind]
+        if kind == "rough":
+            g = gr[i]
+        elif kind == "albedo":
+            g = ga[i, c]
+        else:
+            g = ge.reshape(-1, 3)[i, c]
         results[kind].append((float(g), fd))
         print(f"{kind:6s} {i:6d}.{c}  adjoint {g: .6e}  fd {fd: .6e}", flush=True)
     for kind, rows in results.items():
@@ -127,9 +132,10 @@ def test_autograd_optimisation_descends(setup):
               p0["env"].clone().requires_grad_(True)]
     opt = torch.optim.Adam(params, lr=0.02)
     losses = []
+    # one fixed sample per pixel (stream 1, the target's): a deterministic per-seed objective
     for it in range(12):
         opt.zero_grad()
-        img = nag.render(r, 1 + it, *params)
+        img = nag.render(r, 1, *params)
         loss = ((img - target) ** 2).mean()
         loss.backward()
         assert all(torch.isfinite(p.grad).all() for p in params)
@@ -140,7 +146,8 @@ def test_autograd_optimisation_descends(setup):
             params[2].clamp_(0.0, None)
         losses.append(float(loss))
     print("losses", [f"{v:.4f}" for v in losses])
-    assert losses[-1] < losses[0]
+    assert losses[-1] < 0.95 * losses[0]
+    assert sum(b < a for a, b in zip(losses, losses[1:])) >= len(losses) - 3
 
 
 def test_adjoint_rejects_clamp_and_mis(setup):
