@@ -180,6 +180,27 @@ def bench_adjoint(passes=10):
             "fwd_ms": round(fwd, 4), "bwd_ms": round(bwd, 4), "bwd_over_fwd": round(bwd / fwd, 3)}
 
 
+def pmc_probe():
+    """Short run for hardware counter passes (rocprofv3 --pmc): one wavefront
+    step of the headline workload, then a 1 GiB device-to-device copy whose
+    counter readings calibrate the byte counters (bytes moved are known)."""
+    import torch  # device memory for the calibration copy
+    from noor_amd import scenes
+    from noor_amd.renderer import Renderer
+    wl = scenes.terrain()
+    r = Renderer.from_workload(wl)
+    r.render_wavefront(spp=wl.spp, frame=1)
+    r.synchronize()
+    r.close()
+    n = 1 << 28  # floats: 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"pmc probe: one wavefront step, then a copy of {n * 4} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +217,11 @@ def main():
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(pixels shard trivially; see the sharding design); run with --gpus 1", file=sys.stderr)

@@ -496,31 +496,47 @@ __device__ __forceinline__ void cswap(float& a, float& b, uint32_t& ra, uint32_t
     rb = ub;
 }
 
-// Two-level closest hit (noor/cuda/bvh.cuh:142-191) over the wide BVH.  Ties
-// in t are broken by (prim_id, instance); light nodes win only when strictly
-// closer.  Hit children are followed nearest first.
+// Two-level closest hit (noor/cuda/bvh.cuh:142-191) over the wide BVH, as a
+// resumable state machine: closest_steps() advances a traversal by at most
+// `max_steps` node or leaf visits and returns true when it has finished, so a
+// persistent kernel can hand a finished lane a new ray while the others go
+// on.  Ties in t are broken by (prim_id, instance); light nodes win only when
+// strictly closer.  Hit children are followed nearest first.
+struct ClosestTrav {
+    Ray ray, lray;
+    uint32_t cur, ins;
+    int sp;
+    bool hit;
+    Hit h;
+};
+
+__device__ __forceinline__ void closest_begin(const DevScene& S, const Ray& r, ClosestTrav& t) {
+    t.ray = r;
+    t.lray = r;
+    t.cur = S.wroot;
+    t.ins = 0;
+    t.sp = 0;
+    t.hit = false;
+    t.h.kind = 0;
+    t.h.prim = 0xFFFFFFFFu;
+    t.h.key = 0xFFFFFFFFu;
+    t.h.inst = 0xFFFFFFFFu;
+    t.h.t = 0.0f;
+    t.h.u = t.h.v = 0.0f;
+}
+
 template <bool STATS>
-__device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const TravStack& st, Counters& c,
-                                            bool& overflow) {
-    int sp = 0;
-    Ray lray = ray;
-    uint32_t ins_idx = 0;
-    bool hit = false;
-    h.kind = 0;
-    h.prim = 0xFFFFFFFFu;
-    h.key = 0xFFFFFFFFu;
-    h.inst = 0xFFFFFFFFu;
-    h.t = 0.0f;
-    h.u = h.v = 0.0f;
-    uint32_t cur = S.wroot;
-    for (;;) {
-        if (cur & REF_TLAS) lray = ray;
+__device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t, const TravStack& st, Counters& c,
+                                              bool& overflow, int max_steps) {
+    for (int step = 0; step < max_steps; ++step) {
+        const uint32_t cur = t.cur;
+        if (cur & REF_TLAS) t.lray = t.ray;
         const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
         if (kind == REF_INNER) {
             if (STATS) c.nodes++;
             const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
             float tn[4];
-            const uint32_t m = box4_hit(lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
+            const uint32_t m = box4_hit(t.lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
             if (m != 0u) {
                 float d0 = (m & 1u) ? tn[0] : K_INF, d1 = (m & 2u) ? tn[1] : K_INF;
                 float d2 = (m & 4u) ? tn[2] : K_INF, d3 = (m & 8u) ? tn[3] : K_INF;
@@ -531,94 +547,120 @@ __device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h,
                 cswap(d1, d3, r1, r3);
                 cswap(d1, d2, r1, r2);
                 const int k = __popc(m);
-                if (sp + k - 1 > st.depth) {
+                if (t.sp + k - 1 > st.depth) {
                     overflow = true;
-                    break;
+                    return true;
                 }
-                if (k > 3) st.at(sp++) = r3;
-                if (k > 2) st.at(sp++) = r2;
-                if (k > 1) st.at(sp++) = r1;
-                cur = r0;
+                if (k > 3) st.at(t.sp++) = r3;
+                if (k > 2) st.at(t.sp++) = r2;
+                if (k > 1) st.at(t.sp++) = r1;
+                t.cur = r0;
                 continue;
             }
         } else if (kind == REF_LEAF) {
             const uint32_t start = cur & REF_LEAF_START_MASK;
             const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
             for (uint32_t ti = start; ti < end; ++ti) {
-                float t, u, v;
-                if (!tri_test<STATS>(S, lray, ti, t, u, v, c)) continue;
+                float tt, u, v;
+                if (!tri_test<STATS>(S, t.lray, ti, tt, u, v, c)) continue;
                 const uint32_t key = S.prim_ids[ti];
                 bool take = false;
-                if (t < ray.tmax) take = true;
-                else if (t == ray.tmax) {
-                    if (!hit) take = true;
-                    else if (key < h.key || (key == h.key && ins_idx < h.inst)) take = true;
+                if (tt < t.ray.tmax) take = true;
+                else if (tt == t.ray.tmax) {
+                    if (!t.hit) take = true;
+                    else if (key < t.h.key || (key == t.h.key && t.ins < t.h.inst)) take = true;
                 }
                 if (take) {
-                    hit = true;
-                    ray.tmax = t;
-                    lray.tmax = t;
-                    h.t = t;
-                    h.u = u;
-                    h.v = v;
-                    h.prim = ti;
-                    h.key = key;
-                    h.inst = ins_idx;
-                    h.kind = 1;
+                    t.hit = true;
+                    t.ray.tmax = tt;
+                    t.lray.tmax = tt;
+                    t.h.t = tt;
+                    t.h.u = u;
+                    t.h.v = v;
+                    t.h.prim = ti;
+                    t.h.key = key;
+                    t.h.inst = t.ins;
+                    t.h.kind = 1;
                 }
             }
         } else if (kind == REF_INST) {
-            ins_idx = cur & REF_INDEX_MASK;
+            t.ins = cur & REF_INDEX_MASK;
             if (STATS) c.inst++;
-            const Xf T = xf_load(S.xf + 9 * (size_t)ins_idx);
-            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
-            cur = S.inst_root[ins_idx];
-            if (cur != REF_EMPTY) continue;
+            const Xf T = xf_load(S.xf + 9 * (size_t)t.ins);
+            t.lray = mkray(xf_pt(T, t.ray.o), xf_vec(T, t.ray.d), t.ray.tmax);
+            t.cur = S.inst_root[t.ins];
+            if (t.cur != REF_EMPTY) continue;
         } else {
             const uint32_t li = cur & REF_INDEX_MASK;
-            Ray tr = ray;
-            if (light_node_intersect(S, li, tr) && tr.tmax < ray.tmax) {
-                ray.tmax = tr.tmax;
-                hit = true;
-                h.t = tr.tmax;
-                h.kind = 2;
-                h.inst = li;
-                h.key = 0xFFFFFFFFu;
-                h.prim = 0xFFFFFFFFu;
-                h.u = h.v = 0.0f;
+            Ray tr = t.ray;
+            if (light_node_intersect(S, li, tr) && tr.tmax < t.ray.tmax) {
+                t.ray.tmax = tr.tmax;
+                t.hit = true;
+                t.h.t = tr.tmax;
+                t.h.kind = 2;
+                t.h.inst = li;
+                t.h.key = 0xFFFFFFFFu;
+                t.h.prim = 0xFFFFFFFFu;
+                t.h.u = t.h.v = 0.0f;
             }
         }
-        if (sp == 0) break;
-        cur = st.at(--sp);
+        if (t.sp == 0) return true;
+        t.cur = st.at(--t.sp);
     }
-    return hit;
+    return false;
 }
 
-// Any hit (noor/cuda/bvh.cuh:193-234) over the wide BVH.
 template <bool STATS>
-__device__ __forceinline__ bool bvh_any(const DevScene& S, const Ray& ray, const TravStack& st, Counters& c,
-                                        bool& overflow) {
-    int sp = 0;
-    Ray lray = ray;
-    uint32_t cur = S.wroot;
-    for (;;) {
-        if (cur & REF_TLAS) lray = ray;
+__device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const TravStack& st, Counters& c,
+                                            bool& overflow) {
+    ClosestTrav t;
+    closest_begin(S, ray, t);
+    while (!closest_steps<STATS>(S, t, st, c, overflow, 1 << 30)) {
+    }
+    ray.tmax = t.ray.tmax;
+    h = t.h;
+    return t.hit;
+}
+
+// Any hit (noor/cuda/bvh.cuh:193-234) over the wide BVH, resumable like the
+// closest-hit traversal.
+struct AnyTrav {
+    Ray ray, lray;
+    uint32_t cur;
+    int sp;
+    bool occluded;
+};
+
+__device__ __forceinline__ void any_begin(const DevScene& S, const Ray& r, AnyTrav& t) {
+    t.ray = r;
+    t.lray = r;
+    t.cur = S.wroot;
+    t.sp = 0;
+    t.occluded = false;
+}
+
+template <bool STATS>
+__device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const TravStack& st, Counters& c,
+                                          bool& overflow, int max_steps) {
+    for (int step = 0; step < max_steps; ++step) {
+        const uint32_t cur = t.cur;
+        if (cur & REF_TLAS) t.lray = t.ray;
         const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
         if (kind == REF_INNER) {
             if (STATS) c.nodes++;
             const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
             float tn[4];
-            uint32_t m = box4_hit(lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
+            uint32_t m = box4_hit(t.lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
             if (m != 0u) {
                 const int k = __popc(m);
-                if (sp + k - 1 > st.depth) {
+                if (t.sp + k - 1 > st.depth) {
                     overflow = true;
-                    return false;
+                    return true;
                 }
-                cur = ref_of(n.ref, __ffs(m) - 1);
+                t.cur = ref_of(n.ref, __ffs(m) - 1);
                 m &= m - 1u;
                 while (m) {
-                    st.at(sp++) = ref_of(n.ref, __ffs(m) - 1);
+                    st.at(t.sp++) = ref_of(n.ref, __ffs(m) - 1);
                     m &= m - 1u;
                 }
                 continue;
@@ -627,24 +669,40 @@ __device__ __forceinline__ bool bvh_any(const DevScene& S, const Ray& ray, const
             const uint32_t start = cur & REF_LEAF_START_MASK;
             const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
             for (uint32_t ti = start; ti < end; ++ti) {
-                float t, u, v;
-                if (tri_test<STATS>(S, lray, ti, t, u, v, c)) return true;
+                float tt, u, v;
+                if (tri_test<STATS>(S, t.lray, ti, tt, u, v, c)) {
+                    t.occluded = true;
+                    return true;
+                }
             }
         } else if (kind == REF_INST) {
             const uint32_t ins = cur & REF_INDEX_MASK;
             if (STATS) c.inst++;
             const Xf T = xf_load(S.xf + 9 * (size_t)ins);
-            lray = mkray(xf_pt(T, ray.o), xf_vec(T, ray.d), ray.tmax);
-            cur = S.inst_root[ins];
-            if (cur != REF_EMPTY) continue;
+            t.lray = mkray(xf_pt(T, t.ray.o), xf_vec(T, t.ray.d), t.ray.tmax);
+            t.cur = S.inst_root[ins];
+            if (t.cur != REF_EMPTY) continue;
         } else {
-            Ray tr = ray;
-            if (light_node_intersect(S, cur & REF_INDEX_MASK, tr)) return true;
+            Ray tr = t.ray;
+            if (light_node_intersect(S, cur & REF_INDEX_MASK, tr)) {
+                t.occluded = true;
+                return true;
+            }
         }
-        if (sp == 0) break;
-        cur = st.at(--sp);
+        if (t.sp == 0) return true;
+        t.cur = st.at(--t.sp);
     }
     return false;
+}
+
+template <bool STATS>
+__device__ __forceinline__ bool bvh_any(const DevScene& S, const Ray& ray, const TravStack& st, Counters& c,
+                                        bool& overflow) {
+    AnyTrav t;
+    any_begin(S, ray, t);
+    while (!any_steps<STATS>(S, t, st, c, overflow, 1 << 30)) {
+    }
+    return t.occluded;
 }
 
 // ---------------------------------------------------------------------------------------------

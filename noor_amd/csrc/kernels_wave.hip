@@ -6,13 +6,15 @@
 //   k_regen   retires the paths that ended in the previous iteration (their
 //             radiance goes to the item's entry of the sample buffer) and
 //             starts the next items in the freed slots           -> ray queue
-//   k_extend  closest hit; persistent waves pull 64-ray batches; each hit is
-//             binned by material class with one atomic per class per wave
+//   k_extend  closest hit; persistent waves, and a lane that finishes its
+//             ray takes the next one from the queue (half a wave at a time),
+//             so lanes do not idle behind the longest ray of a batch
+//   k_bin     bins the hits by material class (block-aggregated appends)
 //   k_shade   walks the class queues in class order, so each wave shades one
 //             material; runs shade_bounce(), the per-bounce function of the
 //             megakernel: next rays -> ray queue, NEE rays -> shadow queue,
 //             ended paths -> free queue
-//   k_shadow  any hit for the NEE rays (persistent), resolved into L
+//   k_shadow  any hit for the NEE rays (persistent, lane refill), resolved into L
 // Keeping the pool full means every launch has work for the whole GPU (a
 // layer-by-layer wavefront spends most of its time in nearly empty launches
 // of the late bounces).  When every item has retired, k_finish folds each
@@ -34,6 +36,7 @@ constexpr int kClasses = 6;  // miss, emitter, lambert, conductor, dielectric, o
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 constexpr int kIters = 8;    // iterations per graph replay (even: each replay starts at parity 0)
 constexpr uint64_t kMaxGroupItems = 1ull << 27;  // sample-buffer entries per group of layers
+constexpr int kRefill = 32;  // idle lanes of a wave that trigger a refill from the ray queue
 
 struct ViewDesc {
     noor_camera cam;
@@ -71,6 +74,7 @@ struct WaveBuffers {
     uint32_t capacity = 0;
     int block = 64;
     int stack_lds = 0;
+    int steps = 16;            // traversal steps between refills of the persistent kernels
     uint32_t sb_items = 0;     // sample buffer entries
     uint32_t max_views = 0, max_segments = 0;
     // path state (one slot per path in flight)
@@ -85,6 +89,7 @@ struct WaveBuffers {
     float4* rd = nullptr;
     float4* hit = nullptr;     // t, u, v, prim
     uint2* hit2 = nullptr;     // inst, kind
+    uint32_t* cls = nullptr;   // material class of the hit (k_extend -> k_bin)
     float4* nee_a = nullptr;   // candidate.xyz
     float4* nee_beta = nullptr;
     float4* sh_o = nullptr;    // shadow ray origin, tmax
@@ -110,18 +115,6 @@ struct WaveBuffers {
 // helpers
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
-
-// wave-aggregated append: returns this lane's slot in `queue` (only for lanes with pred)
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    const uint32_t lane = lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (mask != 0 && (int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader < 0 ? 0 : leader, 64);
-    const unsigned long long below = mask & ((1ull << lane) - 1ull);
-    return base + (uint32_t)__popcll(below);
-}
 
 // Block-aggregated appends to three queues at once (one atomic per queue per
 // block): lane results are this lane's slot in each queue where its flag is
@@ -283,6 +276,26 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WaveBuffers B, int cur)
     B.q_ray[cur][c0 + t] = p;
 }
 
+// Refill of a persistent wave: when at least kRefill lanes are idle (or all
+// are), one atomic hands the idle lanes consecutive queue entries.  Returns
+// this lane's entry, or UINT32_MAX.  `exhausted` becomes true (wave-uniform)
+// once the queue has run out.
+__device__ __forceinline__ uint32_t refill(uint32_t* work, uint32_t count, bool have, bool& exhausted) {
+    if (exhausted) return 0xFFFFFFFFu;
+    const unsigned long long idle = __ballot(!have);
+    const int n = __popcll(idle);
+    if (n < kRefill && n < 64) return 0xFFFFFFFFu;
+    const uint32_t lane = lane_id();
+    const int leader = __ffsll((long long)idle) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(work, (uint32_t)n);
+    base = __shfl(base, leader, 64);
+    if (base + (uint32_t)n >= count) exhausted = true;
+    if (have) return 0xFFFFFFFFu;
+    const uint32_t idx = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+    return idx < count ? idx : 0xFFFFFFFFu;
+}
+
 template <bool STATS>
 __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur, DevStats* stats) {
     const DevScene& S = *Sp;
@@ -292,32 +305,25 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
     const uint32_t count = ctl->ray_count[cur] + ctl->new_rays[cur];
     const uint32_t* q = B.q_ray[cur];
     Counters cnt{0, 0, 0, 0};
-    bool overflow = false;
+    bool overflow = false, have = false, exhausted = false;
     unsigned long long ext = 0;
+    uint32_t p = 0;
+    ClosestTrav t;
     for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&ctl->work_extend, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= count) break;
-        const uint32_t i = base + lane_id();
-        const bool active = i < count;
-        uint32_t p = 0;
-        int cls = -1;
-        if (active) {
-            p = q[i];
+        const uint32_t idx = refill(&ctl->work_extend, count, have, exhausted);
+        if (idx != 0xFFFFFFFFu) {
+            p = q[idx];
             const float4 o = B.ro[p], d = B.rd[p];
-            Ray ray = mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, K_INF);
-            Hit h;
+            closest_begin(S, mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, K_INF), t);
+            have = true;
             ++ext;
-            const bool hit = bvh_closest<STATS>(S, ray, h, st, cnt, overflow);
-            B.hit[p] = make_float4(h.t, h.u, h.v, __uint_as_float(h.prim));
-            B.hit2[p] = make_uint2(h.inst, hit ? h.kind : 0u);
-            cls = material_class(S, hit, h);
         }
-        for (int c = 0; c < kClasses; ++c) {
-            const bool mine = (cls == c);
-            const uint32_t slot = wave_append(&ctl->class_count[c], mine);
-            if (mine) B.q_class[(size_t)c * B.capacity + slot] = p;
+        if (__ballot(have) == 0ull) break;
+        if (have && closest_steps<STATS>(S, t, st, cnt, overflow, B.steps)) {
+            B.hit[p] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.prim));
+            B.hit2[p] = make_uint2(t.h.inst, t.hit ? t.h.kind : 0u);
+            B.cls[p] = (uint32_t)material_class(S, t.hit, t.h);
+            have = false;
         }
     }
     unsigned long long e = ext;
@@ -326,6 +332,50 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
     if (STATS) flush4(stats->ext, cnt);
     const unsigned long long ov = __ballot(overflow);
     if (lane_id() == 0 && ov) atomicAdd(&stats->overflow, 1ull);
+}
+
+// Material-class binning of the extension queue, in queue order: one LDS
+// histogram and one atomic per class per block.
+__global__ void k_bin(WaveBuffers B, int cur) {
+    WaveCtl* ctl = B.ctl;
+    const uint32_t count = ctl->ray_count[cur] + ctl->new_rays[cur];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= count) return;  // whole block idle (uniform per block)
+    __shared__ uint32_t cnt[kClasses][16];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const bool valid = t < count;
+    const uint32_t p = valid ? B.q_ray[cur][t] : 0u;
+    const int cls = valid ? (int)B.cls[p] : -1;
+    unsigned long long m[kClasses];
+#pragma unroll
+    for (int c = 0; c < kClasses; ++c) {
+        m[c] = __ballot(cls == c);
+        if (lane == 0) cnt[c][wave] = (uint32_t)__popcll(m[c]);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses) {
+        const int c = threadIdx.x;
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t n = cnt[c][w];
+            cnt[c][w] = tot;
+            tot += n;
+        }
+        const uint32_t base = tot ? atomicAdd(&ctl->class_count[c], tot) : 0u;
+        for (uint32_t w = 0; w < nw; ++w) cnt[c][w] += base;
+    }
+    __syncthreads();
+    if (cls < 0) return;
+    unsigned long long mine = 0;
+    uint32_t base = 0;
+#pragma unroll
+    for (int c = 0; c < kClasses; ++c)
+        if (c == cls) {
+            mine = m[c];
+            base = cnt[c][wave];
+        }
+    const uint32_t slot = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
+    B.q_class[(size_t)cls * B.capacity + slot] = p;
 }
 
 __global__ void k_shade(const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
@@ -402,20 +452,21 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
     const TravStack st = make_stack(S, smem);
     const uint32_t count = ctl->shadow_count;
     Counters cnt{0, 0, 0, 0};
-    bool overflow = false;
+    bool overflow = false, have = false, exhausted = false;
     unsigned long long shd = 0;
+    uint32_t p = 0;
+    AnyTrav t;
     for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&ctl->work_shadow, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= count) break;
-        const uint32_t i = base + lane_id();
-        if (i < count) {
-            const uint32_t p = B.q_shadow[i];
+        const uint32_t idx = refill(&ctl->work_shadow, count, have, exhausted);
+        if (idx != 0xFFFFFFFFu) {
+            p = B.q_shadow[idx];
             const float4 o = B.sh_o[p], d = B.sh_d[p];
-            const Ray ray = mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, o.w);
+            any_begin(S, mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, o.w), t);
+            have = true;
             ++shd;
-            const bool occ = bvh_any<STATS>(S, ray, st, cnt, overflow);
+        }
+        if (__ballot(have) == 0ull) break;
+        if (have && any_steps<STATS>(S, t, st, cnt, overflow, B.steps)) {
             Nee nee;
             const float4 a = B.nee_a[p], bt = B.nee_beta[p];
             nee.a = f3{a.x, a.y, a.z};
@@ -426,8 +477,9 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
             PathState ps;
             const float4 L = B.L[p];
             ps.L = f3{L.x, L.y, L.z};
-            resolve_nee(ps, nee, occ, false);
+            resolve_nee(ps, nee, t.occluded, false);
             B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+            have = false;
         }
     }
     unsigned long long v = shd;
@@ -488,6 +540,7 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
     WA(B.rd, cap);
     WA(B.hit, cap);
     WA(B.hit2, cap);
+    WA(B.cls, cap);
     WA(B.nee_a, cap);
     WA(B.nee_beta, cap);
     WA(B.sh_o, cap);
@@ -539,6 +592,9 @@ static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
             hipLaunchKernelGGL(k_extend<true>, gp, bp, lds, c.stream, c.d_scene, B, cur, c.d_stats);
         else
             hipLaunchKernelGGL(k_extend<false>, gp, bp, lds, c.stream, c.d_scene, B, cur, c.d_stats);
+        T.end(c.stream);
+        T.begin(c.stream, "wave_bin");
+        hipLaunchKernelGGL(k_bin, g1, b1, 0, c.stream, B, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_shade");
         hipLaunchKernelGGL(k_shade, g1, b1, 0, c.stream, c.d_scene, B, cur);
