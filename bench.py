@@ -38,6 +38,10 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s + spp/s @1080p (wavefront vs megakernel); bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
 WIDE_NODE_BYTES = 112  # four child boxes (96 B) + four child references (16 B) per wide-node visit
+# HBM bytes per dispatch of the dominant kernel on the default terrain workload,
+# from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB
+# calibration copy (commands and figures in profiles/pmc_traffic_terrain_1080p.md)
+MEASURED_HBM_BYTES = {("terrain", 1920, 1080, 32, "wave_extend"): 1234954063}
 
 
 def log(msg):
@@ -303,8 +307,11 @@ def main():
     avg_s = ms / 1e3 / launches
     per_launch = bytes_kernel[dom] / launches
     achieved = per_launch / avg_s / 1e9
+    traffic = None
+    if args.queue == 0 and args.block == 64:
+        traffic = MEASURED_HBM_BYTES.get((args.workload, wl.camera.w, wl.camera.h, spp, dom))
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches}
     mega_roof = None
     if "megakernel" in kt:
