@@ -88,7 +88,8 @@ uint32_t stack_need(const noor_payload_view* P, uint32_t ni, std::vector<int32_t
 }
 
 const uint32_t kSupportedMaterials = NOOR_MAT_DIFFUSE | NOOR_MAT_TRANSLUCENT | NOOR_MAT_GLOSSY | NOOR_MAT_MIRROR |
-                                     NOOR_MAT_GLASS | NOOR_MAT_METAL | NOOR_MAT_EMITTER;
+                                     NOOR_MAT_GLASS | NOOR_MAT_METAL | NOOR_MAT_EMITTER | NOOR_MAT_ROUGHGLASS |
+                                     NOOR_MAT_SUBSTRATE | NOOR_MAT_CLEARCOAT | NOOR_MAT_SHADOWCATCHER;
 
 noor_status validate(Context* c, const noor_payload_view* P) {
     if (P->num_triangles && (!P->attribute_indices || !P->wald_ax_nu_nv_nd || !P->wald_bnu_bnv_au_av ||
@@ -397,6 +398,9 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
             d.rough[0] = tmp[0];
             d.rough[1] = tmp[1];
             d.coat_ior = mt.coat_ior;
+            d.coat_weight = mt.coat_weight;
+            d.coat_thickness = mt.coat_thickness;
+            d.coat_sigma = mt.coat_sigma;
             d.type = mt.type;
         }
         if ((st = dev_upload(c, (noor::DevMat**)&S.mats, mats.data(), mats.size())) != NOOR_OK) return bail(st);

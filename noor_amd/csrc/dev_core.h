@@ -31,8 +31,8 @@ namespace noor {
 // BxDFType (noor/cuda/path_tracer.cuh:104-117)
 enum : int { B_REFLECTION = 1, B_TRANSMISSION = 2, B_DIFFUSE = 4, B_GLOSSY = 8, B_SPECULAR = 16,
              B_CONDUCTOR = 32, B_DIELECTRIC = 64, B_NOOP = 128, B_ALL = 255 };
-enum : int { L_LAMBERT_R = 1, L_SPEC_R_NOOP = 2, L_MICRO_R_DIEL = 4, L_MICRO_R_COND = 5,
-             L_LAMBERT_T = 6, L_FRESNEL_SPEC = 10 };
+enum : int { L_SHADOWCATCHER = 0, L_LAMBERT_R = 1, L_SPEC_R_NOOP = 2, L_MICRO_R_DIEL = 4, L_MICRO_R_COND = 5,
+             L_LAMBERT_T = 6, L_MICRO_T = 8, L_FRESNEL_BLEND = 9, L_FRESNEL_SPEC = 10, L_CLEARCOAT = 11 };
 
 constexpr uint32_t M_EMITTER_ANY = NOOR_MAT_EMITTER | NOOR_MAT_MESHLIGHT;
 constexpr uint32_t M_TRANSPARENT = NOOR_MAT_TRANSLUCENT | NOOR_MAT_GLASS | NOOR_MAT_ROUGHGLASS;
@@ -90,6 +90,7 @@ struct DevMat {
     float diffuse[3], specular[3], transmission[3], emittance[3], eta[3], k[3];
     float rough[2];
     float coat_ior;
+    float coat_weight, coat_thickness, coat_sigma;
     uint32_t type;
 };
 
@@ -878,6 +879,7 @@ struct Isect {
     f3 p, wo, wi, gn, sn, ss, ts;
     float u, v;
     uint32_t tri, ins, mat, type;
+    f3 catcher;  // shadow catcher: the sky behind the surface, sky(-wo)
 };
 
 __device__ __forceinline__ bool is_emitter(const Isect& I) { return (I.type & M_EMITTER_ANY) != 0; }
@@ -941,6 +943,8 @@ __device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, co
     const float sign = dot3(cross3(n, ss), dpdv) < 0.0f ? -1.0f : 1.0f;
     I.ss = ss;
     I.ts = nrm3s(scl3(cross3(n, ss), sign));
+    // CudaShadowCatcher::Eval reads the sky behind the surface (noor/cuda/bxdf.cuh:210-216)
+    if (I.type & NOOR_MAT_SHADOWCATCHER) I.catcher = sky_eval(S, scl3(I.wo, -1.0f));
 }
 
 __device__ __forceinline__ f3 to_local(const Isect& I, f3 v) { return f3{dot3(v, I.ss), dot3(v, I.ts), dot3(v, I.sn)}; }
@@ -1096,6 +1100,10 @@ __device__ __forceinline__ void make_bsdf(uint32_t mtype, Bsdf& b) {
     const Lobe mcond{L_MICRO_R_COND, B_CONDUCTOR | B_REFLECTION | B_GLOSSY};
     const Lobe fspec{L_FRESNEL_SPEC, B_REFLECTION | B_TRANSMISSION | B_SPECULAR | B_DIELECTRIC};
     const Lobe mirror{L_SPEC_R_NOOP, B_NOOP | B_REFLECTION | B_SPECULAR};
+    const Lobe catcher{L_SHADOWCATCHER, B_REFLECTION | B_TRANSMISSION | B_DIFFUSE};
+    const Lobe mtrans{L_MICRO_T, B_TRANSMISSION | B_GLOSSY | B_DIELECTRIC};
+    const Lobe blend{L_FRESNEL_BLEND, B_REFLECTION | B_GLOSSY | B_DIELECTRIC};
+    const Lobe coat{L_CLEARCOAT, B_GLOSSY | B_REFLECTION | B_SPECULAR};
     switch (t) {
         case NOOR_MAT_DIFFUSE: b.l[b.n++] = lam; break;
         case NOOR_MAT_TRANSLUCENT: b.l[b.n++] = lam; b.l[b.n++] = lamt; break;
@@ -1103,6 +1111,10 @@ __device__ __forceinline__ void make_bsdf(uint32_t mtype, Bsdf& b) {
         case NOOR_MAT_GLOSSY: b.l[b.n++] = lam; b.l[b.n++] = mdiel; break;
         case NOOR_MAT_METAL: b.l[b.n++] = mcond; break;
         case NOOR_MAT_MIRROR: b.l[b.n++] = mirror; break;
+        case NOOR_MAT_SHADOWCATCHER: b.l[b.n++] = catcher; break;
+        case NOOR_MAT_ROUGHGLASS: b.l[b.n++] = mdiel; b.l[b.n++] = mtrans; break;
+        case NOOR_MAT_SUBSTRATE: b.l[b.n++] = blend; break;
+        case NOOR_MAT_CLEARCOAT: b.l[b.n++] = coat; break;
         default: break;
     }
 }
@@ -1133,24 +1145,6 @@ __device__ __forceinline__ float micro_pdf(const DevMat& m, f3 wo, f3 wi) {
     const Ggx g = ggx_make(m);
     return ggx_pdf(g, wo, wh) / (4.0f * dot3(wo, wh));
 }
-__device__ __forceinline__ f3 lobe_eval(const Lobe& L, const DevMat& m, f3 wo, f3 wi) {
-    switch (L.id) {
-        case L_LAMBERT_R:
-        case L_LAMBERT_T: return scl3(ld3(m.diffuse), K_INV_PI);
-        case L_MICRO_R_DIEL:
-        case L_MICRO_R_COND: return micro_eval(L.type, m, wo, wi);
-        default: return f3{0.0f, 0.0f, 0.0f};
-    }
-}
-__device__ __forceinline__ float lobe_pdf(const Lobe& L, const DevMat& m, f3 wo, f3 wi) {
-    switch (L.id) {
-        case L_LAMBERT_R: return same_hemi(wo, wi) ? fabsf(wi.z) * K_INV_PI : 0.0f;
-        case L_LAMBERT_T: return !same_hemi(wo, wi) ? fabsf(wi.z) * K_INV_PI : 0.0f;
-        case L_MICRO_R_DIEL:
-        case L_MICRO_R_COND: return micro_pdf(m, wo, wi);
-        default: return 0.0f;
-    }
-}
 __device__ __forceinline__ f3 refl_z(f3 wo) { return f3{-wo.x, -wo.y, wo.z}; }
 __device__ __forceinline__ bool refract_(f3 wi, f3 n, float eta, f3& wt) {
     const float cosI = dot3(n, wi);
@@ -1161,8 +1155,131 @@ __device__ __forceinline__ bool refract_(f3 wi, f3 n, float eta, f3& wt) {
     wt = add3(scl3(wi, -eta), scl3(n, eta * cosI - cosT));
     return true;
 }
-__device__ __forceinline__ f3 lobe_sample(const Lobe& L, const DevMat& m, f3 wo, f3& wi, f2 u, float& pdf,
-                                          int& sampled) {
+__device__ __forceinline__ float pow5r(float v) {  // NOOR::pow5f
+    const float a = v * v;
+    return a * a * v;
+}
+
+// CudaMicrofacetTransmission (noor/cuda/bxdf.cuh:559-667): eta from coat_ior
+__device__ __forceinline__ f3 micro_t_eval(const DevMat& m, f3 wo, f3 wi) {
+    if (same_hemi(wo, wi)) return f3{0.0f, 0.0f, 0.0f};
+    const float cO = wo.z, cI = wi.z;
+    if (cI == 0.0f || cO == 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    const float eta = wo.z > 0.0f ? m.coat_ior / 1.0f : 1.0f / m.coat_ior;
+    f3 wh = nrm3(add3(wo, scl3(wi, eta)));
+    wh = scl3(wh, (float)isign(wh.z));
+    const float F = fr_dielectric(dot3(wo, wh), 1.0f, m.coat_ior);
+    const float sd = dot3(wo, wh) + eta * dot3(wi, wh);
+    const Ggx g = ggx_make(m);
+    const float v = ggx_D(g, wh) * ggx_G(g, wo, wi) * eta * eta * absdot3(wi, wh) * absdot3(wo, wh) / (cI * cO * sd * sd);
+    return scl3(scl3(ld3(m.transmission), 1.0f - F), fabsf(v));
+}
+__device__ __forceinline__ float micro_t_pdf(const DevMat& m, f3 wo, f3 wi) {
+    if (same_hemi(wo, wi)) return 0.0f;
+    const float eta = wo.z > 0.0f ? m.coat_ior / 1.0f : 1.0f / m.coat_ior;
+    const f3 wh = nrm3(add3(wo, scl3(wi, eta)));
+    const float sd = dot3(wo, wh) + eta * dot3(wi, wh);
+    const float dwh = fabsf((eta * eta * dot3(wi, wh)) / (sd * sd));
+    return ggx_pdf(ggx_make(m), wo, wh) * dwh;
+}
+
+// CudaFresnelBlend (noor/cuda/bxdf.cuh:669-752)
+__device__ __forceinline__ f3 blend_eval(const DevMat& m, f3 wo, f3 wi) {
+    const f3 S = ld3(m.specular);
+    f3 d = scl3(ld3(m.diffuse), 28.0f / (23.0f * K_PI));
+    d = mul3(d, sub3(f3{1.0f, 1.0f, 1.0f}, S));
+    d = scl3(d, 1.0f - pow5r(1.0f - .5f * fabsf(wi.z)));
+    d = scl3(d, 1.0f - pow5r(1.0f - .5f * fabsf(wo.z)));
+    f3 wh = add3(wi, wo);
+    if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    wh = nrm3(wh);
+    const Ggx g = ggx_make(m);
+    const float sp = ggx_D(g, wh) / (4.0f * absdot3(wi, wh) * rmax(fabsf(wi.z), fabsf(wo.z)));
+    const f3 schlick = add3(S, scl3(sub3(f3{1.0f, 1.0f, 1.0f}, S), pow5r(1.0f - dot3(wi, wh))));
+    return add3(d, scl3(schlick, sp));
+}
+__device__ __forceinline__ float blend_pdf(const DevMat& m, f3 wo, f3 wi) {
+    if (!same_hemi(wo, wi)) return 0.0f;
+    const f3 wh = nrm3(add3(wo, wi));
+    const float pwh = ggx_pdf(ggx_make(m), wo, wh);
+    return .5f * (fabsf(wi.z) * K_INV_PI + pwh / (4.0f * dot3(wo, wh)));
+}
+
+// CudaClearCoat (noor/cuda/bxdf.cuh:842-940): a conductor microfacet
+// substrate under a dielectric coat of index coat_ior
+__device__ __forceinline__ float coat_absorption(const DevMat& m, f3 wo, f3 wi) {
+    const float sigmaA = m.coat_sigma * m.coat_thickness;
+    if (sigmaA != 0.0f) return nm_expf(-sigmaA * (1.0f / fabsf(wo.z) + 1.0f / fabsf(wi.z)));
+    return 1.0f;
+}
+__device__ __forceinline__ f3 coat_eval(const DevMat& m, f3 wo, f3 wi) {
+    const float eta = 1.0f / m.coat_ior;
+    const float F0 = fr_dielectric(wo.z, 1.0f, m.coat_ior);
+    const float F1 = fr_dielectric(wi.z, 1.0f, m.coat_ior);
+    const f3 n = f3{0.0f, 0.0f, 1.0f};
+    f3 wo_c = f3{0.0f, 0.0f, 0.0f}, wi_c = f3{0.0f, 0.0f, 0.0f};
+    refract_(wo, n, eta, wo_c);
+    refract_(wi, n, eta, wi_c);
+    wo_c = scl3(wo_c, -1.0f);
+    wi_c = scl3(wi_c, -1.0f);
+    const f3 r = scl3(scl3(micro_eval(B_CONDUCTOR | B_REFLECTION | B_GLOSSY, m, wo_c, wi_c), 1.0f - F0), 1.0f - F1);
+    return scl3(r, coat_absorption(m, wo_c, wi_c));
+}
+__device__ __forceinline__ float coat_pdf(const DevMat& m, f3 wo, f3 wi) {
+    const float eta = 1.0f / m.coat_ior;
+    const float F = fr_dielectric(wo.z, 1.0f, m.coat_ior);
+    const f3 n = f3{0.0f, 0.0f, 1.0f};
+    f3 wo_c = f3{0.0f, 0.0f, 0.0f}, wi_c = f3{0.0f, 0.0f, 0.0f};
+    refract_(wo, n, eta, wo_c);
+    refract_(wi, n, eta, wi_c);
+    wo_c = scl3(wo_c, -1.0f);
+    wi_c = scl3(wi_c, -1.0f);
+    const float w = m.coat_weight;
+    const float cp = F * w / (F * w + (1.0f - F) * (1.0f - w));
+    return micro_pdf(m, wo_c, wi_c) * (1.0f - cp) + cp;
+}
+
+__device__ __forceinline__ f3 lobe_eval(const Lobe& L, const Isect& I, const DevMat& m, f3 wo, f3 wi) {
+    switch (L.id) {
+        case L_LAMBERT_R:
+        case L_LAMBERT_T: return scl3(ld3(m.diffuse), K_INV_PI);
+        case L_MICRO_R_DIEL:
+        case L_MICRO_R_COND: return micro_eval(L.type, m, wo, wi);
+        case L_SHADOWCATCHER: return sdiv3(I.catcher, fabsf(wi.z));
+        case L_MICRO_T: return micro_t_eval(m, wo, wi);
+        case L_FRESNEL_BLEND: return blend_eval(m, wo, wi);
+        case L_CLEARCOAT: return coat_eval(m, wo, wi);
+        default: return f3{0.0f, 0.0f, 0.0f};
+    }
+}
+__device__ __forceinline__ float lobe_pdf(const Lobe& L, const DevMat& m, f3 wo, f3 wi) {
+    switch (L.id) {
+        case L_LAMBERT_R: return same_hemi(wo, wi) ? fabsf(wi.z) * K_INV_PI : 0.0f;
+        case L_LAMBERT_T: return !same_hemi(wo, wi) ? fabsf(wi.z) * K_INV_PI : 0.0f;
+        case L_MICRO_R_DIEL:
+        case L_MICRO_R_COND: return micro_pdf(m, wo, wi);
+        case L_SHADOWCATCHER: return 1.0f;
+        case L_MICRO_T: return micro_t_pdf(m, wo, wi);
+        case L_FRESNEL_BLEND: return blend_pdf(m, wo, wi);
+        case L_CLEARCOAT: return coat_pdf(m, wo, wi);
+        default: return 0.0f;
+    }
+}
+// MicrofacetReflection::Sample_f (noor/cuda/bxdf.cuh:519-537)
+__device__ __forceinline__ f3 micro_sample(int ltype, const DevMat& m, f3 wo, f3& wi, f2 u, float& pdf) {
+    if (wo.z == 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    const Ggx g = ggx_make(m);
+    const f3 wh = ggx_sample_wh(g, wo, u);
+    wi = add3(scl3(wo, -1.0f), scl3(wh, 2.0f * dot3(wo, wh)));
+    if (!same_hemi(wo, wi)) {
+        pdf = 0.0f;
+        return f3{0.0f, 0.0f, 0.0f};
+    }
+    pdf = ggx_pdf(g, wo, wh) / (4.0f * dot3(wo, wh));
+    return micro_eval(ltype, m, wo, wi);
+}
+__device__ __forceinline__ f3 lobe_sample(const Lobe& L, const Isect& I, const DevMat& m, f3 wo, f3& wi, f2 u,
+                                          float& pdf, int& sampled) {
     switch (L.id) {
         case L_LAMBERT_R:
         case L_LAMBERT_T: {
@@ -1174,22 +1291,12 @@ __device__ __forceinline__ f3 lobe_sample(const Lobe& L, const DevMat& m, f3 wo,
                 if (wo.z > 0.0f) wi.z *= -1.0f;
             }
             pdf = lobe_pdf(L, m, wo, wi);
-            return lobe_eval(L, m, wo, wi);
+            return lobe_eval(L, I, m, wo, wi);
         }
         case L_MICRO_R_DIEL:
-        case L_MICRO_R_COND: {
+        case L_MICRO_R_COND:
             sampled = L.type;
-            if (wo.z == 0.0f) return f3{0.0f, 0.0f, 0.0f};
-            const Ggx g = ggx_make(m);
-            const f3 wh = ggx_sample_wh(g, wo, u);
-            wi = add3(scl3(wo, -1.0f), scl3(wh, 2.0f * dot3(wo, wh)));
-            if (!same_hemi(wo, wi)) {
-                pdf = 0.0f;
-                return f3{0.0f, 0.0f, 0.0f};
-            }
-            pdf = ggx_pdf(g, wo, wh) / (4.0f * dot3(wo, wh));
-            return micro_eval(L.type, m, wo, wi);
-        }
+            return micro_sample(L.type, m, wo, wi, u, pdf);
         case L_SPEC_R_NOOP: {
             sampled = L.type;
             wi = refl_z(wo);
@@ -1214,6 +1321,90 @@ __device__ __forceinline__ f3 lobe_sample(const Lobe& L, const DevMat& m, f3 wo,
             sampled = B_TRANSMISSION | B_SPECULAR;
             return sdiv3(scl3(scl3(scl3(ld3(m.transmission), eta), eta), 1.0f - F), fabsf(wi.z));
         }
+        case L_SHADOWCATCHER: {
+            // uniformSampleHemisphere, LEFT_HANDED (noor/cuda/utils.cuh:364-372)
+            sampled = L.type;
+            pdf = 1.0f;
+            const float z = u.x;
+            const float r = sqrtf(rmax(0.0f, 1.0f - z * z));
+            const float phi = 2.0f * K_PI * u.y;
+            wi = f3{r * nm_cosf(phi), r * nm_sinf(phi), z};
+            return sdiv3(I.catcher, fabsf(wi.z));
+        }
+        case L_MICRO_T: {
+            sampled = L.type;
+            if (wo.z == 0.0f) return f3{0.0f, 0.0f, 0.0f};
+            const Ggx g = ggx_make(m);
+            f3 wh = ggx_sample_wh(g, wo, u);
+            const float eta = wo.z > 0.0f ? 1.0f / m.coat_ior : m.coat_ior / 1.0f;
+            if (!refract_(wo, wh, eta, wi)) {
+                // total internal reflection: helper_math reflect(i, n) = i - 2 n (n.i)
+                wi = sub3(wo, scl3(scl3(wh, 2.0f), dot3(wh, wo)));
+                wh = scl3(wh, (float)isign(wh.z));
+                const float F = fr_dielectric(dot3(wi, wh), 1.0f, m.coat_ior);
+                const float cO = wo.z, cI = wi.z;
+                const float dg = ggx_D(g, wh) * ggx_G(g, wo, wi);
+                pdf = ggx_pdf(g, wo, wh) / (4.0f * dot3(wo, wh));
+                return sdiv3(scl3(scl3(ld3(m.specular), F), dg), 4.0f * cI * cO);
+            }
+            pdf = micro_t_pdf(m, wo, wi);
+            return micro_t_eval(m, wo, wi);
+        }
+        case L_FRESNEL_BLEND: {
+            sampled = L.type;
+            f2 lu = u;
+            if (lu.x < .5f) {
+                lu.x = rmin(2.0f * lu.x, K_ONE_MINUS_EPS);
+                wi = cosine_hemisphere(lu);
+                wi.z *= (float)isign(wi.z);
+            } else {
+                lu.x = rmin(2.0f * (lu.x - .5f), K_ONE_MINUS_EPS);
+                const f3 wh = ggx_sample_wh(ggx_make(m), wo, lu);
+                wi = add3(scl3(wo, -1.0f), scl3(wh, 2.0f * dot3(wo, wh)));
+                if (!same_hemi(wo, wi)) return f3{0.0f, 0.0f, 0.0f};
+            }
+            pdf = blend_pdf(m, wo, wi);
+            return blend_eval(m, wo, wi);
+        }
+        case L_CLEARCOAT: {
+            const float eta = 1.0f / m.coat_ior;
+            const float F0 = fr_dielectric(wo.z, 1.0f, m.coat_ior);
+            const float w = m.coat_weight;
+            const float ps = F0 * w / (F0 * w + (1.0f - F0) * (1.0f - w));
+            f2 smp = u;
+            bool spec;
+            if (smp.y < ps) {
+                smp.y /= ps;
+                spec = true;
+            } else {
+                smp.y = (smp.y - ps) / (1.0f - ps);
+                spec = false;
+            }
+            if (spec) {
+                // the coat: CudaSpecularReflection(BSDF_DIELECTRIC) (noor/cuda/bxdf.cuh:377-391)
+                wi = refl_z(wo);
+                sampled = B_DIELECTRIC | B_REFLECTION | B_SPECULAR;
+                const float F = fr_dielectric(wi.z, 1.0f, m.coat_ior);
+                pdf = ps;
+                return sdiv3(scl3(ld3(m.diffuse), F), fabsf(wi.z));
+            }
+            const f3 n = f3{0.0f, 0.0f, 1.0f};
+            f3 wo_c = f3{0.0f, 0.0f, 0.0f}, wi_c = f3{0.0f, 0.0f, 0.0f};
+            refract_(wo, n, eta, wo_c);
+            sampled = B_CONDUCTOR | B_REFLECTION | B_GLOSSY;
+            float spdf = 0.0f;
+            f3 r = micro_sample(B_CONDUCTOR | B_REFLECTION | B_GLOSSY, m, scl3(wo_c, -1.0f), wi_c, smp, spdf);
+            const float F1 = fr_dielectric(-wi_c.z, 1.0f, m.coat_ior);
+            refract_(scl3(wi_c, -1.0f), scl3(n, -1.0f), 1.0f / eta, wi);
+            if (F1 == 1.0f) {
+                pdf = 0.0f;
+                return f3{0.0f, 0.0f, 0.0f};
+            }
+            const float k = coat_absorption(m, wo_c, wi_c) * (1.0f - F0) * (1.0f - F1);
+            r = sdiv3(scl3(r, k), fabsf(wi_c.z));
+            pdf = spdf * (1.0f - ps);
+            return r;
+        }
         default: return f3{0.0f, 0.0f, 0.0f};
     }
 }
@@ -1226,7 +1417,7 @@ __device__ __forceinline__ f3 bsdf_eval(const Bsdf& b, const Isect& I, const Dev
     for (int i = 0; i < b.n; ++i)
         if (lobe_matches(b.l[i].type, flags) &&
             ((reflect && (b.l[i].type & B_REFLECTION)) || (!reflect && (b.l[i].type & B_TRANSMISSION))))
-            f = add3(f, lobe_eval(b.l[i], m, wo, wi));
+            f = add3(f, lobe_eval(b.l[i], I, m, wo, wi));
     return f;
 }
 __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
@@ -1272,7 +1463,7 @@ __device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const D
     pdf = 0.0f;
     sampled = L.type;
     f3 wi = f3{0.0f, 0.0f, 0.0f};
-    f3 f = lobe_sample(L, m, wo, wi, ur, pdf, sampled);
+    f3 f = lobe_sample(L, I, m, wo, wi, ur, pdf, sampled);
     if (pdf == 0.0f) {
         sampled = 0;
         return f3{0.0f, 0.0f, 0.0f};
@@ -1288,7 +1479,7 @@ __device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const D
         for (int i = 0; i < b.n; ++i)
             if (i != li && lobe_matches(b.l[i].type, flags) &&
                 ((reflect && (b.l[i].type & B_REFLECTION)) || (!reflect && (b.l[i].type & B_TRANSMISSION))))
-                f = add3(f, lobe_eval(b.l[i], m, wo, wi));
+                f = add3(f, lobe_eval(b.l[i], I, m, wo, wi));
     }
     return f;
 }

@@ -91,11 +91,12 @@ class Payload:
 
 def material(mtype, diffuse=(0.5, 0.5, 0.5), specular=(1.0, 1.0, 1.0), transmission=(1.0, 1.0, 1.0),
              emittance=(0.0, 0.0, 0.0), eta=(1.0, 1.0, 1.0), k=(0.0, 0.0, 0.0), roughness=(0.5, 0.5),
-             coat_ior=1.5, emitter_scale=1.0):
+             coat_ior=1.5, emitter_scale=1.0, coat_weight=0.5, coat_thickness=0.0, coat_sigma=0.0):
     """Material description; textures are created by SceneBuilder.add_material."""
     return dict(type=int(mtype), diffuse=diffuse, specular=specular, transmission=transmission,
                 emittance=emittance, eta=eta, k=k, roughness=roughness, coat_ior=coat_ior,
-                emitter_scale=emitter_scale)
+                emitter_scale=emitter_scale, coat_weight=coat_weight, coat_thickness=coat_thickness,
+                coat_sigma=coat_sigma)
 
 
 class SceneBuilder:
@@ -142,11 +143,11 @@ class SceneBuilder:
         mt.bump_tex_idx = 0
         mt.bumpfactor = 1.0
         mt.emitter_scale = m["emitter_scale"]
-        mt.coat_weight = 1.0
         mt.coat_roughness = 0.001
-        mt.coat_thickness = 1.0
-        mt.coat_sigma = 0.0
         mt.coat_ior = m["coat_ior"]
+        mt.coat_weight = m.get("coat_weight", 0.5)
+        mt.coat_thickness = m.get("coat_thickness", 0.0)
+        mt.coat_sigma = m.get("coat_sigma", 0.0)
         mt.type = m["type"]
         self.materials.append(mt)
         return len(self.materials) - 1

@@ -421,3 +421,35 @@ def batch_views(n=48, seed=7, center=(50.0, 45.0, 50.0), sizes=((256, 256), (512
         cam = make_camera(tuple(eye), tuple(look), (0, 1, 0), np.radians(rng.uniform(35.0, 60.0)), w, h)
         out.append((cam, 1, spp, int(rng.integers(1 << 20))))
     return out
+
+
+def showcase(seed=5, subdiv=4, width=96, height=80, spp=4, bounces=6, rr=3, mis=False,
+             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL):
+    """The secondary materials of the reference's factory (shadow catcher,
+    rough glass, substrate, clear coat) beside glossy and diffuse ones, in the
+    open Cornell geometry under an HDR environment."""
+    sb, rng, meshes, (sp, sf), _ = _cornell_builder(seed, subdiv, with_ceiling=False)
+    mats = [
+        sb.add_material(material(abi.MAT_SHADOWCATCHER)),                                      # floor
+        sb.add_material(material(abi.MAT_SUBSTRATE, diffuse=(0.6, 0.3, 0.2), specular=(0.1, 0.1, 0.1),
+                                 roughness=(0.2, 0.2))),                                        # back wall
+        sb.add_material(material(abi.MAT_CLEARCOAT, diffuse=(1.0, 1.0, 1.0), specular=(1.0, 1.0, 1.0),
+                                 eta=CU_ETA, k=CU_K, roughness=(0.3, 0.3), coat_ior=1.5,
+                                 coat_weight=0.4, coat_thickness=0.5, coat_sigma=0.3)),         # left wall
+        sb.add_material(material(abi.MAT_GLOSSY, diffuse=(0.2, 0.5, 0.3), specular=(0.3, 0.3, 0.3),
+                                 roughness=(0.25, 0.25))),                                      # right wall
+        sb.add_material(material(abi.MAT_ROUGHGLASS, specular=(1.0, 1.0, 1.0), transmission=(0.9, 0.95, 1.0),
+                                 roughness=(0.2, 0.2), coat_ior=1.5)),                          # block
+        sb.add_material(material(abi.MAT_CLEARCOAT, diffuse=(1.0, 1.0, 1.0), specular=(1.0, 1.0, 1.0),
+                                 eta=CU_ETA, k=CU_K, roughness=(0.1, 0.1), coat_ior=1.6,
+                                 coat_weight=0.6)),                                             # sphere
+    ]
+    for k, (pos, tri) in enumerate(meshes):
+        sb.add_mesh(Mesh(pos, tri, material=mats[min(k, 4)]))
+    sb.add_mesh(Mesh(sp, sf, material=mats[5]))
+    sb.set_env(env_map(128, 64, seed + 3, lobes=2, sigma_deg=8.0, peak=20.0))
+    payload = sb.build()
+    cam = make_camera((0.0, 0.0, 0.95), (0.0, -0.1, -1.0), (0, 1, 0), np.radians(60.0), width, height)
+    spec = make_spec(payload, bounces=bounces, rr=rr, skydome_type=abi.SKY_HDR, mis=mis)
+    opt = make_options(rng=rng_kind, compat=compat)
+    return Workload("showcase", payload, cam, spec, opt, None, spp, info=dict(triangles=payload.view.num_triangles))

@@ -201,8 +201,9 @@ def rng_sequence(kind, gid, frame, n):
 
 
 def material_params(diffuse=(0.5, 0.5, 0.5), specular=(1, 1, 1), eta=(1, 1, 1), k=(0, 0, 0), rough=(0.3, 0.3),
-                    coat_ior=1.5, transmission=(1, 1, 1)):
-    return np.array([*diffuse, *specular, *eta, *k, *rough, coat_ior, *transmission], np.float32)
+                    coat_ior=1.5, transmission=(1, 1, 1), coat_weight=0.5, coat_thickness=0.0, coat_sigma=0.0):
+    return np.array([*diffuse, *specular, *eta, *k, *rough, coat_ior, *transmission, coat_weight, coat_thickness,
+                     coat_sigma], np.float32)
 
 
 def bsdf_eval(mtype, params, wo, wi):

@@ -37,6 +37,7 @@ typedef struct orc_material {
     float eta[3], k[3];
     float rough[2];
     float coat_ior;
+    float coat_weight, coat_thickness, coat_sigma;
 } orc_material;
 
 /* 2D piecewise-constant distribution in the reference's single-buffer layout

@@ -28,7 +28,8 @@ void oracle_rng_sequence(int kind, uint32_t gid, uint32_t frame, int n, float* o
 }
 
 static orc_material mat_from(const float* p) {
-    /* p: diffuse[3] specular[3] eta[3] k[3] rough[2] coat_ior transmission[3] */
+    /* p: diffuse[3] specular[3] eta[3] k[3] rough[2] coat_ior transmission[3]
+     *    coat_weight coat_thickness coat_sigma */
     orc_material m;
     memset(&m, 0, sizeof(m));
     memcpy(m.diffuse, p, 12);
@@ -39,6 +40,9 @@ static orc_material mat_from(const float* p) {
     m.rough[1] = p[13];
     m.coat_ior = p[14];
     memcpy(m.transmission, p + 15, 12);
+    m.coat_weight = p[18];
+    m.coat_thickness = p[19];
+    m.coat_sigma = p[20];
     return m;
 }
 

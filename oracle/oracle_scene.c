@@ -265,6 +265,9 @@ static void resolve_material(const noor_payload_view* p, const noor_material* m,
     o->rough[0] = tmp[0];
     o->rough[1] = tmp[1];
     o->coat_ior = m->coat_ior;
+    o->coat_weight = m->coat_weight;
+    o->coat_thickness = m->coat_thickness;
+    o->coat_sigma = m->coat_sigma;
 }
 
 static int bake_sky(orc_scene* s) {

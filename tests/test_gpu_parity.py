@@ -149,3 +149,22 @@ def test_large_scene_megakernel_and_wavefront_vs_oracle(name):
     e1, e2 = mean_rel_err(g, c), mean_rel_err(w.film(), c)
     print(f"{name}: megakernel {e1:.3e} wavefront {e2:.3e}")
     assert e1 <= TOL_IMAGE and e2 <= TOL_IMAGE
+
+
+@pytest.mark.parametrize("mis", [False, True])
+def test_secondary_materials_vs_oracle(mis):
+    """Shadow catcher, rough glass, substrate and clear coat (the rest of the
+    reference's material factory) under an HDR environment: megakernel and,
+    without MIS, the wavefront renderer against the oracle."""
+    wl = scenes.showcase(mis=mis)
+    r = Renderer.from_workload(wl)
+    r.render(spp=4)
+    g = r.film()
+    c, _ = OracleScene.from_workload(wl).render(1, 4)
+    err = mean_rel_err(g, c)
+    print(f"mis={mis}: megakernel vs oracle {err:.3e}, identical {np.mean(np.all(g == c, axis=-1)):.4f}")
+    assert err <= TOL_IMAGE
+    if not mis:
+        w = Renderer.from_workload(wl)
+        w.render_wavefront(spp=4)
+        np.testing.assert_array_equal(w.film(), g)
