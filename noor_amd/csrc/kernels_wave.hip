@@ -378,7 +378,7 @@ __global__ void k_bin(WaveBuffers B, int cur) {
     B.q_class[(size_t)cls * B.capacity + slot] = p;
 }
 
-__global__ void k_shade(const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
+__global__ __launch_bounds__(256) void k_shade(const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
     const DevScene& S = *Sp;
     WaveCtl* ctl = B.ctl;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
