@@ -503,17 +503,25 @@ __device__ __forceinline__ void cswap(float& a, float& b, uint32_t& ra, uint32_t
 // persistent kernel can hand a finished lane a new ray while the others go
 // on.  Ties in t are broken by (prim_id, instance); light nodes win only when
 // strictly closer.  Hit children are followed nearest first.
+// The world ray is kept as origin and direction only; lray is the ray of the
+// level being traversed (world, or object space inside an instance) with its
+// precomputed terms and the current closest distance, and is rebuilt from the
+// world ray on the way back to the top level.  Keeping one full Ray instead of
+// two saves the registers that decide how many waves a CU can keep in flight.
 struct ClosestTrav {
-    Ray ray, lray;
+    f3 o, d;
+    Ray lray;
     uint32_t cur, ins;
     int sp;
-    bool hit;
+    bool hit, world;
     Hit h;
 };
 
 __device__ __forceinline__ void closest_begin(const DevScene& S, const Ray& r, ClosestTrav& t) {
-    t.ray = r;
+    t.o = r.o;
+    t.d = r.d;
     t.lray = r;
+    t.world = true;
     t.cur = S.wroot;
     t.ins = 0;
     t.sp = 0;
@@ -531,7 +539,10 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
                                               bool& overflow, int max_steps) {
     for (int step = 0; step < max_steps; ++step) {
         const uint32_t cur = t.cur;
-        if (cur & REF_TLAS) t.lray = t.ray;
+        if ((cur & REF_TLAS) && !t.world) {
+            t.lray = mkray(t.o, t.d, t.lray.tmax);
+            t.world = true;
+        }
         const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
         if (kind == REF_INNER) {
             if (STATS) c.nodes++;
@@ -566,14 +577,13 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
                 if (!tri_test<STATS>(S, t.lray, ti, tt, u, v, c)) continue;
                 const uint32_t key = S.prim_ids[ti];
                 bool take = false;
-                if (tt < t.ray.tmax) take = true;
-                else if (tt == t.ray.tmax) {
+                if (tt < t.lray.tmax) take = true;
+                else if (tt == t.lray.tmax) {
                     if (!t.hit) take = true;
                     else if (key < t.h.key || (key == t.h.key && t.ins < t.h.inst)) take = true;
                 }
                 if (take) {
                     t.hit = true;
-                    t.ray.tmax = tt;
                     t.lray.tmax = tt;
                     t.h.t = tt;
                     t.h.u = u;
@@ -588,14 +598,16 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
             t.ins = cur & REF_INDEX_MASK;
             if (STATS) c.inst++;
             const Xf T = xf_load(S.xf + 9 * (size_t)t.ins);
-            t.lray = mkray(xf_pt(T, t.ray.o), xf_vec(T, t.ray.d), t.ray.tmax);
+            t.lray = mkray(xf_pt(T, t.o), xf_vec(T, t.d), t.lray.tmax);
+            t.world = false;
             t.cur = S.inst_root[t.ins];
             if (t.cur != REF_EMPTY) continue;
         } else {
+            // light nodes are top-level references: lray is the world ray here
             const uint32_t li = cur & REF_INDEX_MASK;
-            Ray tr = t.ray;
-            if (light_node_intersect(S, li, tr) && tr.tmax < t.ray.tmax) {
-                t.ray.tmax = tr.tmax;
+            Ray tr = t.lray;
+            if (light_node_intersect(S, li, tr) && tr.tmax < t.lray.tmax) {
+                t.lray.tmax = tr.tmax;
                 t.hit = true;
                 t.h.t = tr.tmax;
                 t.h.kind = 2;
@@ -618,7 +630,7 @@ __device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h,
     closest_begin(S, ray, t);
     while (!closest_steps<STATS>(S, t, st, c, overflow, 1 << 30)) {
     }
-    ray.tmax = t.ray.tmax;
+    ray.tmax = t.lray.tmax;
     h = t.h;
     return t.hit;
 }
@@ -626,15 +638,18 @@ __device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h,
 // Any hit (noor/cuda/bvh.cuh:193-234) over the wide BVH, resumable like the
 // closest-hit traversal.
 struct AnyTrav {
-    Ray ray, lray;
+    f3 o, d;  // world ray; lray as in ClosestTrav (tmax fixed)
+    Ray lray;
     uint32_t cur;
     int sp;
-    bool occluded;
+    bool occluded, world;
 };
 
 __device__ __forceinline__ void any_begin(const DevScene& S, const Ray& r, AnyTrav& t) {
-    t.ray = r;
+    t.o = r.o;
+    t.d = r.d;
     t.lray = r;
+    t.world = true;
     t.cur = S.wroot;
     t.sp = 0;
     t.occluded = false;
@@ -645,7 +660,10 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
                                           bool& overflow, int max_steps) {
     for (int step = 0; step < max_steps; ++step) {
         const uint32_t cur = t.cur;
-        if (cur & REF_TLAS) t.lray = t.ray;
+        if ((cur & REF_TLAS) && !t.world) {
+            t.lray = mkray(t.o, t.d, t.lray.tmax);
+            t.world = true;
+        }
         const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
         if (kind == REF_INNER) {
             if (STATS) c.nodes++;
@@ -680,11 +698,12 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
             const uint32_t ins = cur & REF_INDEX_MASK;
             if (STATS) c.inst++;
             const Xf T = xf_load(S.xf + 9 * (size_t)ins);
-            t.lray = mkray(xf_pt(T, t.ray.o), xf_vec(T, t.ray.d), t.ray.tmax);
+            t.lray = mkray(xf_pt(T, t.o), xf_vec(T, t.d), t.lray.tmax);
+            t.world = false;
             t.cur = S.inst_root[ins];
             if (t.cur != REF_EMPTY) continue;
         } else {
-            Ray tr = t.ray;
+            Ray tr = t.lray;
             if (light_node_intersect(S, cur & REF_INDEX_MASK, tr)) {
                 t.occluded = true;
                 return true;

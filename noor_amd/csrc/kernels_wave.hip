@@ -306,7 +306,7 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
     const uint32_t* q = B.q_ray[cur];
     Counters cnt{0, 0, 0, 0};
     bool overflow = false, have = false, exhausted = false;
-    unsigned long long ext = 0;
+    uint32_t ext = 0;  // rays this lane started
     uint32_t p = 0;
     ClosestTrav t;
     for (;;) {
@@ -327,7 +327,7 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
         }
     }
     unsigned long long e = ext;
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);  // 64-bit wave sum
     if (lane_id() == 0 && e) atomicAdd(&stats->extension_rays, e);
     if (STATS) flush4(stats->ext, cnt);
     const unsigned long long ov = __ballot(overflow);
@@ -453,7 +453,7 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
     const uint32_t count = ctl->shadow_count;
     Counters cnt{0, 0, 0, 0};
     bool overflow = false, have = false, exhausted = false;
-    unsigned long long shd = 0;
+    uint32_t shd = 0;  // rays this lane started
     uint32_t p = 0;
     AnyTrav t;
     for (;;) {
