@@ -37,7 +37,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s + spp/s @1080p (wavefront vs megakernel); bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
-WIDE_NODE_BYTES = 112  # four child boxes (96 B) + four child references (16 B) per wide-node visit
+WIDE_NODE_BYTES = 56  # per wide-node visit: origin + steps (16 B), 8-bit child planes (24 B), references (16 B)
 # HBM bytes per dispatch of the dominant kernel on the default terrain workload,
 # from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB
 # calibration copy (commands and figures in profiles/pmc_traffic_terrain_1080p.md)

@@ -330,7 +330,7 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
         S.stack_depth = (int)c->stack_need;
         S.stack_lds = std::min<int>(S.stack_depth, c->stack_lds);
         S.wroot = wb.root;
-        if ((st = dev_upload(c, (noor::WNode**)&S.wbvh, wb.nodes.data(), wb.nodes.size())) != NOOR_OK) return bail(st);
+        if ((st = dev_upload(c, (noor::QNode**)&S.wbvh, wb.qnodes.data(), wb.qnodes.size())) != NOOR_OK) return bail(st);
         if ((st = dev_upload(c, (uint32_t**)&S.inst_root, wb.inst_root.data(), wb.inst_root.size())) != NOOR_OK)
             return bail(st);
         c->wide_nodes = wb.nodes.size();

@@ -261,38 +261,66 @@ __device__ __forceinline__ Xf xf_from(const noor_transform& t) {
 // (CudaBBox::intersect, noor/cuda/bbox.cuh:62-80).  The exit distance and the
 // ray's tmax are widened by 2^-20 relative, more than the rounding of this
 // test and of the triangle tests, so a box holding a hit that a triangle test
-// accepts is always entered.  Which boxes a traversal visits, and in which
-// order, then cannot change the closest hit (ties are ordered by primitive
-// id): the wide BVH returns the hits of the binary BVH it was built from.
-// fmaxf/fminf ignore the NaN of an origin on the slab plane of a zero
-// direction component.  Bit `c` of the result is set when child c is hit;
-// tn[c] is its entry distance.
+// accepts is always entered.  The decoded boxes of a compressed node contain
+// the exact ones, which only adds entered boxes.  Which boxes a traversal
+// visits, and in which order, then cannot change the closest hit (ties are
+// ordered by primitive id): the wide BVH returns the hits of the binary BVH it
+// was built from.  fmaxf/fminf ignore the NaN of an origin on the slab plane
+// of a zero direction component.
 constexpr float BOX_SLACK = 1.00000095367431640625f;  // 1 + 2^-20
 
-__device__ __forceinline__ uint32_t box4_hit(const Ray& r, const uint4& lx, const uint4& ly, const uint4& lz,
-                                             const uint4& hx, const uint4& hy, const uint4& hz, float tn[4]) {
-    const uint4 nx = (r.pn & 1u) ? hx : lx, fx = (r.pn & 1u) ? lx : hx;
-    const uint4 ny = (r.pn & 2u) ? hy : ly, fy = (r.pn & 2u) ? ly : hy;
-    const uint4 nz = (r.pn & 4u) ? hz : lz, fz = (r.pn & 4u) ? lz : hz;
-    const uint32_t NX[4] = {nx.x, nx.y, nx.z, nx.w}, FX[4] = {fx.x, fx.y, fx.z, fx.w};
-    const uint32_t NY[4] = {ny.x, ny.y, ny.z, ny.w}, FY[4] = {fy.x, fy.y, fy.z, fy.w};
-    const uint32_t NZ[4] = {nz.x, nz.y, nz.z, nz.w}, FZ[4] = {fz.x, fz.y, fz.z, fz.w};
+// A compressed node (QNode in wide_bvh.h) as loaded: 56 of its 64 bytes.
+struct WideNode {
+    uint4 a;  // org.xyz, meta
+    uint4 b;  // qlo.xyz, qhi.x
+    uint4 c;  // qhi.yz, ref0, ref1
+    uint2 d;  // ref2, ref3
+    __device__ __forceinline__ uint4 ref() const { return make_uint4(c.z, c.w, d.x, d.y); }
+};
+__device__ __forceinline__ WideNode load_wnode(const DevScene& S, uint32_t idx) {
+    const uint4* p = S.wbvh + (size_t)idx * 4;
+    WideNode n;
+    n.a = p[0];
+    n.b = p[1];
+    n.c = p[2];
+    n.d = *reinterpret_cast<const uint2*>(p + 3);
+    return n;
+}
+// Plane c (byte c of q) of an axis with origin org and step s: the decode the
+// builder checked (quantize in wide_bvh.cpp), exact product, one rounding.
+__device__ __forceinline__ float q_plane(float org, uint32_t q, int c, float s) {
+    const float qs = (float)((q >> (8 * c)) & 0xFFu) * s;
+    return org + qs;
+}
+
+// Slab test of the four children of a compressed node.  Bit c of the result
+// is set when child c is entered; tn[c] is its entry distance.
+__device__ __forceinline__ uint32_t box4_hit(const Ray& r, const WideNode& n, float tn[4]) {
+    const float ox = __uint_as_float(n.a.x), oy = __uint_as_float(n.a.y), oz = __uint_as_float(n.a.z);
+    const uint32_t meta = n.a.w;
+    const float sx = __uint_as_float((meta & 0xFFu) << 23);
+    const float sy = __uint_as_float(((meta >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((meta >> 16) & 0xFFu) << 23);
+    const uint32_t qlx = n.b.x, qly = n.b.y, qlz = n.b.z, qhx = n.b.w, qhy = n.c.x, qhz = n.c.y;
+    const uint32_t nx = (r.pn & 1u) ? qhx : qlx, fx = (r.pn & 1u) ? qlx : qhx;
+    const uint32_t ny = (r.pn & 2u) ? qhy : qly, fy = (r.pn & 2u) ? qly : qhy;
+    const uint32_t nz = (r.pn & 4u) ? qhz : qlz, fz = (r.pn & 4u) ? qlz : qhz;
     const float tmaxs = r.tmax * BOX_SLACK;
     uint32_t mask = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float tx0 = (__uint_as_float(NX[c]) - r.o.x) * r.inv.x;
-        const float tx1 = (__uint_as_float(FX[c]) - r.o.x) * r.inv.x;
-        const float ty0 = (__uint_as_float(NY[c]) - r.o.y) * r.inv.y;
-        const float ty1 = (__uint_as_float(FY[c]) - r.o.y) * r.inv.y;
-        const float tz0 = (__uint_as_float(NZ[c]) - r.o.z) * r.inv.z;
-        const float tz1 = (__uint_as_float(FZ[c]) - r.o.z) * r.inv.z;
+        const float tx0 = (q_plane(ox, nx, c, sx) - r.o.x) * r.inv.x;
+        const float tx1 = (q_plane(ox, fx, c, sx) - r.o.x) * r.inv.x;
+        const float ty0 = (q_plane(oy, ny, c, sy) - r.o.y) * r.inv.y;
+        const float ty1 = (q_plane(oy, fy, c, sy) - r.o.y) * r.inv.y;
+        const float tz0 = (q_plane(oz, nz, c, sz) - r.o.z) * r.inv.z;
+        const float tz1 = (q_plane(oz, fz, c, sz) - r.o.z) * r.inv.z;
         const float t0 = fmaxf(fmaxf(tx0, ty0), tz0);
         const float t1 = fminf(fminf(tx1, ty1), tz1) * BOX_SLACK;
         tn[c] = t0;
         if (t0 <= t1 && t0 <= tmaxs && t1 >= 0.0f) mask |= 1u << c;
     }
-    return mask;
+    return mask & (meta >> 24);
 }
 
 struct Counters {
@@ -469,21 +497,6 @@ __device__ __forceinline__ TravStack make_stack(const DevScene& S, uint32_t* sme
     return t;
 }
 
-struct WideNode {
-    uint4 lx, ly, lz, hx, hy, hz, ref;
-};
-__device__ __forceinline__ WideNode load_wnode(const DevScene& S, uint32_t idx) {
-    const uint4* p = S.wbvh + (size_t)idx * 8;
-    WideNode n;
-    n.lx = p[0];
-    n.ly = p[1];
-    n.lz = p[2];
-    n.hx = p[3];
-    n.hy = p[4];
-    n.hz = p[5];
-    n.ref = p[6];
-    return n;
-}
 __device__ __forceinline__ uint32_t ref_of(const uint4& r, int i) {
     return i == 0 ? r.x : (i == 1 ? r.y : (i == 2 ? r.z : r.w));
 }
@@ -548,11 +561,12 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
             if (STATS) c.nodes++;
             const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
             float tn[4];
-            const uint32_t m = box4_hit(t.lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
+            const uint32_t m = box4_hit(t.lray, n, tn);
             if (m != 0u) {
                 float d0 = (m & 1u) ? tn[0] : K_INF, d1 = (m & 2u) ? tn[1] : K_INF;
                 float d2 = (m & 4u) ? tn[2] : K_INF, d3 = (m & 8u) ? tn[3] : K_INF;
-                uint32_t r0 = n.ref.x, r1 = n.ref.y, r2 = n.ref.z, r3 = n.ref.w;
+                const uint4 rf = n.ref();
+                uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
                 cswap(d0, d1, r0, r1);
                 cswap(d2, d3, r2, r3);
                 cswap(d0, d2, r0, r2);
@@ -669,17 +683,18 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
             if (STATS) c.nodes++;
             const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
             float tn[4];
-            uint32_t m = box4_hit(t.lray, n.lx, n.ly, n.lz, n.hx, n.hy, n.hz, tn);
+            uint32_t m = box4_hit(t.lray, n, tn);
             if (m != 0u) {
                 const int k = __popc(m);
                 if (t.sp + k - 1 > st.depth) {
                     overflow = true;
                     return true;
                 }
-                t.cur = ref_of(n.ref, __ffs(m) - 1);
+                const uint4 rf = n.ref();
+                t.cur = ref_of(rf, __ffs(m) - 1);
                 m &= m - 1u;
                 while (m) {
-                    st.at(t.sp++) = ref_of(n.ref, __ffs(m) - 1);
+                    st.at(t.sp++) = ref_of(rf, __ffs(m) - 1);
                     m &= m - 1u;
                 }
                 continue;

@@ -18,6 +18,79 @@
 namespace noor {
 namespace {
 
+// The decode the GPU evaluates (box4_hit in dev_core.h): with contraction off
+// both compute the product exactly (q has 8 bits, s is a power of two) and
+// round the sum once.
+float q_decode(float org, uint32_t q, float s) {
+    const float qs = (float)q * s;
+    return org + qs;
+}
+
+// Encodes one axis of one node: the smallest power-of-two step for which every
+// slot's rounded-out planes fit in 8 bits.  Returns false if none does.
+bool quantize_axis(const WNode& n, int a, uint32_t valid, float& org, uint32_t& ebyte, uint32_t& qlo, uint32_t& qhi) {
+    float lo = std::numeric_limits<float>::infinity(), hi = -std::numeric_limits<float>::infinity();
+    for (int c = 0; c < 4; ++c)
+        if (valid & (1u << c)) {
+            lo = std::min(lo, n.lo[a][c]);
+            hi = std::max(hi, n.hi[a][c]);
+        }
+    org = valid ? lo : 0.0f;
+    if (!std::isfinite(org) || (valid && !std::isfinite(hi))) return false;
+    const double ext = valid ? (double)hi - (double)lo : 0.0;
+    int k = -126;  // smallest normal step
+    if (ext > 0.0) k = std::max(k, (int)std::ceil(std::log2(ext / 255.0)));
+    for (; k <= 127; ++k) {
+        const float s = std::ldexp(1.0f, k);
+        uint32_t L = 0, H = 0;
+        bool ok = true;
+        for (int c = 0; c < 4 && ok; ++c) {
+            if (!(valid & (1u << c))) continue;
+            const float cl = n.lo[a][c], ch = n.hi[a][c];
+            double ql = std::floor(((double)cl - (double)org) / (double)s);
+            int64_t q = (int64_t)std::max(0.0, std::min(255.0, ql));
+            while (q > 0 && q_decode(org, (uint32_t)q, s) > cl) --q;
+            if (q_decode(org, (uint32_t)q, s) > cl) ok = false;
+            int64_t r = (int64_t)std::max(0.0, std::ceil(((double)ch - (double)org) / (double)s));
+            if (r > 255) {
+                ok = false;
+                break;
+            }
+            while (r <= 255 && q_decode(org, (uint32_t)r, s) < ch) ++r;
+            if (r > 255 || r < q) {
+                ok = false;
+                break;
+            }
+            L |= (uint32_t)q << (8 * c);
+            H |= (uint32_t)r << (8 * c);
+        }
+        if (ok) {
+            ebyte = (uint32_t)(k + 127);
+            qlo = L;
+            qhi = H;
+            return true;
+        }
+    }
+    return false;
+}
+
+std::string quantize(const WNode& n, QNode& q) {
+    std::memset(&q, 0, sizeof(q));
+    uint32_t valid = 0;
+    for (int c = 0; c < 4; ++c) {
+        q.ref[c] = n.ref[c];
+        if (n.ref[c] != REF_EMPTY) valid |= 1u << c;
+    }
+    q.meta = valid << 24;
+    for (int a = 0; a < 3; ++a) {
+        uint32_t e = 0;
+        if (!quantize_axis(n, a, valid, q.org[a], e, q.qlo[a], q.qhi[a]))
+            return "a node box cannot be quantised (non-finite or too large a range)";
+        q.meta |= e << (8 * a);
+    }
+    return std::string();
+}
+
 enum BinKind { B_LIGHT, B_INST, B_MLEAF, B_TLEAF, B_INNER };
 
 struct Child {
@@ -38,6 +111,8 @@ class Builder {
         if (W_.nodes.size() >= REF_INDEX_MASK) return "too many wide BVH nodes";
         need_.assign(W_.nodes.size(), -1);
         W_.stack_need = std::max<uint32_t>(1u, need(W_.root));
+        W_.qnodes.resize(W_.nodes.size());
+        for (size_t i = 0; i < W_.nodes.size() && err_.empty(); ++i) err_ = quantize(W_.nodes[i], W_.qnodes[i]);
         return err_;
     }
 
@@ -368,6 +443,24 @@ std::string check_wide_bvh(const noor_payload_view* P, const WideBvh& W) {
         if (want != got) return "a BLAS does not cover its triangles exactly once";
     }
     if (W.stack_need < 1) return "stack bound";
+    // compressed nodes: same references, decoded boxes around the exact ones
+    if (W.qnodes.size() != nn) return "compressed node count";
+    for (size_t i = 0; i < nn; ++i) {
+        const WNode& n = W.nodes[i];
+        const QNode& q = W.qnodes[i];
+        for (int c = 0; c < 4; ++c) {
+            if (q.ref[c] != n.ref[c]) return "compressed node reference";
+            const bool valid = ((q.meta >> 24) >> c) & 1u;
+            if (valid != (n.ref[c] != REF_EMPTY)) return "compressed node slot mask";
+            if (!valid) continue;
+            for (int a = 0; a < 3; ++a) {
+                const float s = std::ldexp(1.0f, (int)((q.meta >> (8 * a)) & 0xFFu) - 127);
+                const float lo = q_decode(q.org[a], (q.qlo[a] >> (8 * c)) & 0xFFu, s);
+                const float hi = q_decode(q.org[a], (q.qhi[a] >> (8 * c)) & 0xFFu, s);
+                if (!(lo <= n.lo[a][c] && hi >= n.hi[a][c])) return "compressed box does not contain its child";
+            }
+        }
+    }
     return std::string();
 }
 

@@ -30,8 +30,9 @@ constexpr uint32_t REF_LEAF_COUNT_SHIFT = 26;
 constexpr uint32_t REF_LEAF_MAX = 8u;
 constexpr uint32_t REF_EMPTY = 0xFFFFFFFFu;
 
-// 128 bytes, one cache line: SoA boxes of the four children (an empty slot has
-// lo = +inf, hi = -inf and never passes the slab test), then their references.
+// Host form of a wide node: SoA boxes of the four children (an empty slot has
+// lo = +inf, hi = -inf), then their references.  The builder and the checks
+// work on this form; the GPU reads the compressed QNode below.
 struct alignas(128) WNode {
     float lo[3][4];
     float hi[3][4];
@@ -39,6 +40,27 @@ struct alignas(128) WNode {
     uint32_t pad[4];
 };
 static_assert(sizeof(WNode) == 128, "WNode is one 128-byte line");
+
+// Device form, 64 bytes (56 read per visit): the four child boxes quantised
+// to 8 bits per plane on a grid local to the node (as in Ylitie, Karras &
+// Laine, "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide
+// BVHs", HPG 2017).  On axis a a plane decodes to
+//     org[a] + (float)q * 2^(e[a] - 127)        (binary32, no contraction)
+// with e[a] = byte a of `meta`.  The encoder rounds lower planes down and upper
+// planes up and checks the decoded values, so every decoded box contains its
+// child's exact box; with the conservative slab test the traversal therefore
+// returns the same hits as over the exact boxes.  Byte 3 of `meta` has bit c
+// set when slot c holds a child.  Words: [org.xyz, meta] [qlo.xyz, qhi.x]
+// [qhi.yz, ref0, ref1] [ref2, ref3, pad, pad]; slot c is byte c of each q word.
+struct alignas(64) QNode {
+    float org[3];
+    uint32_t meta;
+    uint32_t qlo[3];
+    uint32_t qhi[3];
+    uint32_t ref[4];
+    uint32_t pad[2];
+};
+static_assert(sizeof(QNode) == 64, "QNode is 64 bytes");
 
 }  // namespace noor
 
@@ -50,7 +72,8 @@ static_assert(sizeof(WNode) == 128, "WNode is one 128-byte line");
 namespace noor {
 
 struct WideBvh {
-    std::vector<WNode> nodes;
+    std::vector<WNode> nodes;         // exact boxes (host)
+    std::vector<QNode> qnodes;        // the same nodes, compressed (uploaded)
     std::vector<uint32_t> inst_root;  // per transform: reference of its BLAS root
     uint32_t root = REF_EMPTY;
     uint32_t stack_need = 0;          // most stack entries any traversal can hold

@@ -174,7 +174,8 @@ def test_context_calls_reject_null_context():
 def test_wide_bvh_covers_every_triangle_once(name):
     """The four-wide BVH of the traversal kernels, built on the host from the
     payload's binary BVH: exact triangle coverage per BLAS, nested boxes,
-    leaf boxes around their triangles."""
+    leaf boxes around their triangles, and each 8-bit quantised box (as the
+    GPU decodes it) containing the exact box of its child."""
     kw = dict(width=16, height=16)
     if name == "terrain":
         kw.update(grid=200, n_spheres=50, env_w=64, env_h=32)
