@@ -28,6 +28,27 @@ namespace noor {
 #define K_ONE_MINUS_EPS (1.0f - 1.1920928955078125e-07f)
 #define K_INF __builtin_huge_valf()
 
+// Address-space qualified pointers: loads through them compile to global
+// (or LDS) instructions instead of flat ones.  Device allocations are global.
+#define NOOR_GLOBAL __attribute__((address_space(1)))
+#define NOOR_LDS __attribute__((address_space(3)))
+typedef uint32_t noor_v4u __attribute__((ext_vector_type(4)));
+typedef uint32_t noor_v2u __attribute__((ext_vector_type(2)));
+typedef float noor_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 ldg(const uint4* p) {
+    const noor_v4u v = *(const NOOR_GLOBAL noor_v4u*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ldg(const uint2* p) {
+    const noor_v2u v = *(const NOOR_GLOBAL noor_v2u*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ float4 ldg(const float4* p) {
+    const noor_v4f v = *(const NOOR_GLOBAL noor_v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t ldg(const uint32_t* p) { return *(const NOOR_GLOBAL uint32_t*)p; }
+
 // BxDFType (noor/cuda/path_tracer.cuh:104-117)
 enum : int { B_REFLECTION = 1, B_TRANSMISSION = 2, B_DIFFUSE = 4, B_GLOSSY = 8, B_SPECULAR = 16,
              B_CONDUCTOR = 32, B_DIELECTRIC = 64, B_NOOP = 128, B_ALL = 255 };
@@ -235,7 +256,7 @@ struct Xf {
     f3 r0, r1, r2, t;
 };
 __device__ __forceinline__ Xf xf_load(const float4* rows) {
-    const float4 a = rows[0], b = rows[1], c = rows[2];
+    const float4 a = ldg(rows), b = ldg(rows + 1), c = ldg(rows + 2);
     Xf x;
     x.r0 = f3{a.x, a.y, a.z};
     x.r1 = f3{b.x, b.y, b.z};
@@ -280,10 +301,13 @@ struct WideNode {
 __device__ __forceinline__ WideNode load_wnode(const DevScene& S, uint32_t idx) {
     const uint4* p = S.wbvh + (size_t)idx * 4;
     WideNode n;
-    n.a = p[0];
-    n.b = p[1];
-    n.c = p[2];
-    n.d = *reinterpret_cast<const uint2*>(p + 3);
+    n.a = ldg(p);
+    n.b = ldg(p + 1);
+    n.c = ldg(p + 2);
+    n.d = ldg(reinterpret_cast<const uint2*>(p + 3));
+    // the references are needed only when a child is entered, but loading
+    // them then would add a second dependent fetch: issue them with the boxes
+    asm volatile("" ::"v"(n.d.x), "v"(n.d.y), "v"(n.c.z), "v"(n.c.w));
     return n;
 }
 // Plane c (byte c of q) of an axis with origin org and step s: the decode the
@@ -330,8 +354,8 @@ struct Counters {
 // Wald projection test (noor/cuda/triangle.cuh:82-141)
 __device__ __forceinline__ bool tri_wald(const DevScene& S, const Ray& r, uint32_t i, float& to, float& uo,
                                          float& vo, Counters& c) {
-    const uint4 a = S.wald0[i];
-    const uint4 cc = S.wald2[i];
+    const uint4 a = ldg(S.wald0 + i);
+    const uint4 cc = ldg(S.wald2 + i);
     if (cc.z == 1u) return false;
     const uint32_t k = a.x;
     const uint32_t ku = (k == 0u) ? 1u : ((k == 1u) ? 2u : 0u);
@@ -343,7 +367,7 @@ __device__ __forceinline__ bool tri_wald(const DevScene& S, const Ray& r, uint32
     if (denom == 0.0f) return false;
     const float t = (nd - ro_k - nu * ro_u - nv * ro_v) / denom;
     if (t < 0.0f || t > r.tmax) return false;
-    const uint4 b = S.wald1[i];
+    const uint4 b = ldg(S.wald1 + i);
     c.tri2++;
     const float hu = ro_u + t * rd_u - __uint_as_float(b.z);
     const float hv = ro_v + t * rd_v - __uint_as_float(b.w);
@@ -361,7 +385,7 @@ __device__ __forceinline__ bool tri_wald(const DevScene& S, const Ray& r, uint32
 __device__ __forceinline__ bool tri_watertight(const DevScene& S, const Ray& r, uint32_t i, float& to,
                                                float& uo, float& vo) {
     const float4* T = S.tri + 3 * (size_t)i;
-    const float4 p0 = T[0], p1 = T[1], p2 = T[2];
+    const float4 p0 = ldg(T), p1 = ldg(T + 1), p2 = ldg(T + 2);
     const f3 A = sub3(f3{p0.x, p0.y, p0.z}, r.o);
     const f3 B = sub3(f3{p1.x, p1.y, p1.z}, r.o);
     const f3 C = sub3(f3{p2.x, p2.y, p2.z}, r.o);
@@ -474,24 +498,32 @@ struct Hit {
 // banks); deeper entries, which few rays reach, live in device memory laid
 // out the same way over all lanes of the launch.  `depth` is the exact bound
 // of the BVH, so the stack cannot overflow.
+// The two parts are typed by address space so that entries move with LDS and
+// global instructions: through a generic pointer every access would be a flat
+// one, and each wait for an LDS entry would also wait for the node loads in
+// flight.
 struct TravStack {
-    uint32_t* lds;
+    NOOR_LDS uint32_t* lds;
     int lstride;
     int K;
-    uint32_t* glob;
+    NOOR_GLOBAL uint32_t* glob;
     uint32_t gstride;
     int depth;
-    __device__ __forceinline__ uint32_t& at(int d) const {
+    __device__ __forceinline__ void put(int d, uint32_t v) const {
+        if (d < K) lds[d * lstride] = v;
+        else glob[(size_t)(d - K) * gstride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(int d) const {
         return d < K ? lds[d * lstride] : glob[(size_t)(d - K) * gstride];
     }
 };
 // Stack of this lane; the launch must use at most S.gstack_lanes lanes.
 __device__ __forceinline__ TravStack make_stack(const DevScene& S, uint32_t* smem) {
     TravStack t;
-    t.lds = smem + threadIdx.x;
+    t.lds = (NOOR_LDS uint32_t*)(smem + threadIdx.x);
     t.lstride = (int)blockDim.x;
     t.K = S.stack_lds;
-    t.glob = S.gstack + (blockIdx.x * blockDim.x + threadIdx.x);
+    t.glob = (NOOR_GLOBAL uint32_t*)(S.gstack + (blockIdx.x * blockDim.x + threadIdx.x));
     t.gstride = S.gstack_lanes;
     t.depth = S.stack_depth;
     return t;
@@ -577,9 +609,9 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
                     overflow = true;
                     return true;
                 }
-                if (k > 3) st.at(t.sp++) = r3;
-                if (k > 2) st.at(t.sp++) = r2;
-                if (k > 1) st.at(t.sp++) = r1;
+                if (k > 3) st.put(t.sp++, r3);
+                if (k > 2) st.put(t.sp++, r2);
+                if (k > 1) st.put(t.sp++, r1);
                 t.cur = r0;
                 continue;
             }
@@ -589,7 +621,7 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
             for (uint32_t ti = start; ti < end; ++ti) {
                 float tt, u, v;
                 if (!tri_test<STATS>(S, t.lray, ti, tt, u, v, c)) continue;
-                const uint32_t key = S.prim_ids[ti];
+                const uint32_t key = ldg(S.prim_ids + ti);
                 bool take = false;
                 if (tt < t.lray.tmax) take = true;
                 else if (tt == t.lray.tmax) {
@@ -614,7 +646,7 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
             const Xf T = xf_load(S.xf + 9 * (size_t)t.ins);
             t.lray = mkray(xf_pt(T, t.o), xf_vec(T, t.d), t.lray.tmax);
             t.world = false;
-            t.cur = S.inst_root[t.ins];
+            t.cur = ldg(S.inst_root + t.ins);
             if (t.cur != REF_EMPTY) continue;
         } else {
             // light nodes are top-level references: lray is the world ray here
@@ -632,7 +664,7 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
             }
         }
         if (t.sp == 0) return true;
-        t.cur = st.at(--t.sp);
+        t.cur = st.get(--t.sp);
     }
     return false;
 }
@@ -694,7 +726,7 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
                 t.cur = ref_of(rf, __ffs(m) - 1);
                 m &= m - 1u;
                 while (m) {
-                    st.at(t.sp++) = ref_of(rf, __ffs(m) - 1);
+                    st.put(t.sp++, ref_of(rf, __ffs(m) - 1));
                     m &= m - 1u;
                 }
                 continue;
@@ -715,7 +747,7 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
             const Xf T = xf_load(S.xf + 9 * (size_t)ins);
             t.lray = mkray(xf_pt(T, t.o), xf_vec(T, t.d), t.lray.tmax);
             t.world = false;
-            t.cur = S.inst_root[ins];
+            t.cur = ldg(S.inst_root + ins);
             if (t.cur != REF_EMPTY) continue;
         } else {
             Ray tr = t.lray;
@@ -725,7 +757,7 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
             }
         }
         if (t.sp == 0) return true;
-        t.cur = st.at(--t.sp);
+        t.cur = st.get(--t.sp);
     }
     return false;
 }
