@@ -48,6 +48,12 @@ __device__ __forceinline__ float4 ldg(const float4* p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ uint32_t ldg(const uint32_t* p) { return *(const NOOR_GLOBAL uint32_t*)p; }
+__device__ __forceinline__ float ldg(const float* p) { return *(const NOOR_GLOBAL float*)p; }
+typedef float noor_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ldg(const float2* p) {
+    const noor_v2f v = *(const NOOR_GLOBAL noor_v2f*)p;
+    return make_float2(v.x, v.y);
+}
 
 // BxDFType (noor/cuda/path_tracer.cuh:104-117)
 enum : int { B_REFLECTION = 1, B_TRANSMISSION = 2, B_DIFFUSE = 4, B_GLOSSY = 8, B_SPECULAR = 16,
@@ -812,13 +818,13 @@ __device__ __forceinline__ float d_cdf(const DevScene& S, int row, int col) {
     const long idx = (long)row * (S.dw + 1) + col;
     const long n = (long)S.dh * (S.dw + 2) + 1;
     if (idx < 0 || idx >= n) return 0.0f;
-    return S.dcdf[idx];
+    return ldg(S.dcdf + idx);
 }
 __device__ __forceinline__ float d_func(const DevScene& S, int row, int col) {
     const long idx = (long)row * (S.dw + 1) + col;
     const long n = (long)S.dh * (S.dw + 2) + 1;
     if (idx < 0 || idx >= n) return 0.0f;
-    return S.dfunc[idx];
+    return ldg(S.dfunc + idx);
 }
 __device__ __forceinline__ float d_funcint(const DevScene& S, int row) {
     return d_func(S, row, row == S.dh ? S.dh : S.dw);
@@ -880,8 +886,8 @@ __device__ __forceinline__ f3 tex_bilinear(const float4* tex, int w, int h, floa
     const int i1 = ix + 1 < 0 ? 0 : (ix + 1 > w - 1 ? w - 1 : ix + 1);
     const int j0 = iy < 0 ? 0 : (iy > h - 1 ? h - 1 : iy);
     const int j1 = iy + 1 < 0 ? 0 : (iy + 1 > h - 1 ? h - 1 : iy + 1);
-    const float4 t00 = tex[(size_t)j0 * w + i0], t10 = tex[(size_t)j0 * w + i1];
-    const float4 t01 = tex[(size_t)j1 * w + i0], t11 = tex[(size_t)j1 * w + i1];
+    const float4 t00 = ldg(tex + (size_t)j0 * w + i0), t10 = ldg(tex + (size_t)j0 * w + i1);
+    const float4 t01 = ldg(tex + (size_t)j1 * w + i0), t11 = ldg(tex + (size_t)j1 * w + i1);
     const float bx = 1.0f - ax, by = 1.0f - ay;
     f3 r;
     r.x = by * (bx * t00.x + ax * t10.x) + ay * (bx * t01.x + ax * t11.x);
@@ -967,14 +973,14 @@ __device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, co
         return;
     }
     I.tri = h.prim;
-    I.type = S.wald2[h.prim].w;
-    const uint4 ai = S.attr[I.tri];
+    I.type = ldg(S.wald2 + h.prim).w;
+    const uint4 ai = ldg(S.attr + I.tri);
     I.mat = ai.w;
     if (I.type & M_EMITTER_ANY) return;
-    const float2 q0 = S.uv[ai.x], q1 = S.uv[ai.y], q2 = S.uv[ai.z];
+    const float2 q0 = ldg(S.uv + ai.x), q1 = ldg(S.uv + ai.y), q2 = ldg(S.uv + ai.z);
     const f2 uv0 = f2{q0.x, 1.0f - q0.y}, uv1 = f2{q1.x, 1.0f - q1.y}, uv2 = f2{q2.x, 1.0f - q2.y};
-    const f3 p0 = fromf4(S.vert[ai.x]), p1 = fromf4(S.vert[ai.y]), p2 = fromf4(S.vert[ai.z]);
-    const f3 n0 = fromf4(S.nrm[ai.x]), n1 = fromf4(S.nrm[ai.y]), n2 = fromf4(S.nrm[ai.z]);
+    const f3 p0 = fromf4(ldg(S.vert + ai.x)), p1 = fromf4(ldg(S.vert + ai.y)), p2 = fromf4(ldg(S.vert + ai.z));
+    const f3 n0 = fromf4(ldg(S.nrm + ai.x)), n1 = fromf4(ldg(S.nrm + ai.y)), n2 = fromf4(ldg(S.nrm + ai.z));
     const float4* X = S.xf + 9 * (size_t)I.ins;
     Xf T = xf_load(X + 6);
     T.t = f3{0.0f, 0.0f, 0.0f};
