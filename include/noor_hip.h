@@ -232,7 +232,12 @@ enum { NOOR_TRI_WATERTIGHT = 0, NOOR_TRI_WALD = 1 };
 #define NOOR_COMPAT_CDF_N_PLUS_1  0x2u  /* sampleContinuous2D divides by W+1, H+1 (noor/cuda/distribution2d.cuh:331-333) */
 #define NOOR_COMPAT_CLAMP         0x4u  /* clamp each sample to [0,100] (noor/cuda/path_tracer.cu:73) */
 #define NOOR_COMPAT_WASTED_DRAWS  0x8u  /* two unused draws in scatter (noor/cuda/scatter.cuh:34) */
-#define NOOR_COMPAT_ALL           0xFu
+#define NOOR_COMPAT_MIS_LIGHT_COUNT 0x10u /* MIS: the BSDF-sampled NEE term is not scaled by the light count
+                                             (noor/cuda/direct.cuh:69-113; sample_Li is, noor/cuda/lightimp.cuh:415-456) */
+#define NOOR_COMPAT_MIS_SELF_SHADOW 0x20u /* MIS: the shadow ray to a BSDF-sampled light point ends on the light
+                                             and can be blocked by it (noor/cuda/direct.cuh:98-104,
+                                             noor/cuda/intersection.cuh:201-207) */
+#define NOOR_COMPAT_ALL           0x3Fu
 
 typedef struct noor_options {
     int32_t rng;            /* NOOR_RNG_* */

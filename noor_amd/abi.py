@@ -234,14 +234,15 @@ MAT_DIFFUSE, MAT_TRANSLUCENT, MAT_GLOSSY, MAT_MIRROR = 1, 4, 8, 16
 MAT_GLASS, MAT_ROUGHGLASS, MAT_METAL, MAT_EMITTER = 32, 64, 128, 512
 MAT_SUBSTRATE, MAT_CLEARCOAT, MAT_SHADOWCATCHER = 256, 1024, 2048
 MAT_MESHLIGHT = 1 << 14
-LIGHT_AREA, LIGHT_INFINITE = 8, 16
+LIGHT_POINT, LIGHT_SPOT, LIGHT_DISTANT, LIGHT_AREA, LIGHT_INFINITE = 1, 2, 4, 8, 16
 SHAPE_QUAD, SHAPE_SPHERE, SHAPE_DISK = 0, 1, 2
 SKY_HDR, SKY_PHYSICAL, SKY_CONSTANT = 0, 1, 2
 CAMERA_PERSP, CAMERA_ORTHO, CAMERA_ENV = 0, 1, 2
 RNG_TEA_LCG, RNG_PCG32 = 0, 1
 TRI_WATERTIGHT, TRI_WALD = 0, 1
 COMPAT_QUAD_PDF, COMPAT_CDF_N_PLUS_1, COMPAT_CLAMP, COMPAT_WASTED_DRAWS = 1, 2, 4, 8
-COMPAT_ALL = 15
+COMPAT_MIS_LIGHT_COUNT, COMPAT_MIS_SELF_SHADOW = 16, 32
+COMPAT_ALL = 63
 LIGHT_SKY_MASK = 0x20
 
 OK, ERR_INVALID_ARG, ERR_INVALID_PAYLOAD, ERR_HIP, ERR_OUT_OF_MEMORY = 0, 1, 2, 3, 4

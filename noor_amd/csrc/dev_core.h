@@ -2102,8 +2102,15 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
                 const f3 o = dot3(wi, I.gn) >= 0.0f ? add3(I.p, scl3(I.gn, S.bias)) : sub3(I.p, scl3(I.gn, S.bias));
                 Ray sr = mkray(o, wi, 2.0f * S.world_radius);
                 if (light_intersect(S, sr, light_idx)) {
-                    const Vis v2 = mkvis(I.p, ray_at(sr, sr.tmax));
+                    // the reference aims the shadow ray at the light point itself, where the
+                    // light can block it; the corrected mode stops one bias short
+                    const float te = (S.compat & NOOR_COMPAT_MIS_SELF_SHADOW) ? sr.tmax : sr.tmax - S.bias;
+                    const Vis v2 = mkvis(I.p, ray_at(sr, te));
                     nee.b = sdiv3(scl3(mul3(light_Le(S, wi, light_idx), f), sw), spdf);
+                    // the light was chosen with probability 1 / (light count); the
+                    // reference leaves this half of the estimate unscaled
+                    if (!(S.compat & NOOR_COMPAT_MIS_LIGHT_COUNT))
+                        nee.b = scl3(nee.b, (float)(S.sky_enabled ? S.num_lights : S.num_lights - 1));
                     nee.rb = shadow_ray_vis(S, I, v2);
                     nee.nb = 1;
                 }

@@ -106,6 +106,9 @@ class SceneBuilder:
         self.materials = []
         self.textures = [np.full((1, 1, 4), 0.5, np.float32)]  # texture 0: default grey
         self.area_lights = []
+        self.point_lights = []
+        self.spot_lights = []
+        self.distant_lights = []
         self.env = None
 
     def add_mesh(self, mesh):
@@ -174,6 +177,76 @@ class SceneBuilder:
         self.area_lights.append(L)
         return len(self.area_lights) - 1
 
+    def _area(self, ke, shape_type, center, radius, normal=(0.0, 0.0, 1.0), area=0.0, two_sided=False):
+        L = abi.AreaLight()
+        L.ke = abi.float3(*map(float, ke))
+        L.type = abi.LIGHT_AREA
+        s = L.shape
+        s.center = abi.float3(*map(float, center))
+        s.u = abi.float3(0.0, 0.0, 0.0)
+        s.v = abi.float3(0.0, 0.0, 0.0)
+        s.n = abi.float3(*map(float, normal))
+        s.radius = float(radius)
+        s.radius2 = float(np.float32(radius) * np.float32(radius))
+        s.area = float(np.float32(area))
+        s.type = shape_type
+        L.two_sided = 1 if two_sided else 0
+        L.power = abi.float3(0, 0, 0)
+        self.area_lights.append(L)
+        return len(self.area_lights) - 1
+
+    def add_sphere_light(self, center, radius, ke):
+        """Spherical area light, emitting on its whole surface (CudaShape SPHERE,
+        noor/cuda/shape.cuh; sampled by cone, noor/cuda/geometry.cuh:48-85)."""
+        return self._area(ke, abi.SHAPE_SPHERE, center, radius, area=4.0 * np.pi * radius * radius)
+
+    def add_disk_light(self, center, normal, radius, ke, two_sided=False):
+        """Disk area light facing `normal` (CudaShape DISK, noor/cuda/geometry.cuh:86-103)."""
+        n = np.asarray(normal, np.float64)
+        n = n / np.linalg.norm(n)
+        return self._area(ke, abi.SHAPE_DISK, center, radius, normal=n, area=np.pi * radius * radius,
+                          two_sided=two_sided)
+
+    def add_point_light(self, position, ke):
+        """Point light of intensity ke (CudaPointLight, noor/cuda/lightimp.cuh:190-205)."""
+        L = abi.PointLight()
+        L.ke = abi.float3(*map(float, ke))
+        L.power = abi.float3(0, 0, 0)
+        L.type = abi.LIGHT_POINT
+        L.position = abi.float3(*map(float, position))
+        self.point_lights.append(L)
+        return len(self.point_lights) - 1
+
+    def add_spot_light(self, position, direction, cos_inner, cos_outer, ke):
+        """Spot light (CudaSpotLight, noor/cuda/light.cuh:185-226): full intensity
+        where dot(direction, w) >= cos_inner for w from the lit point towards the
+        light, none below cos_outer, a quartic falloff between."""
+        d = np.asarray(direction, np.float64)
+        d = d / np.linalg.norm(d)
+        L = abi.SpotLight()
+        L.ke = abi.float3(*map(float, ke))
+        L.power = abi.float3(0, 0, 0)
+        L.type = abi.LIGHT_SPOT
+        L.position = abi.float3(*map(float, position))
+        L.direction = abi.float3(*map(float, d))
+        L.cosinner = float(cos_inner)
+        L.cosouter = float(cos_outer)
+        self.spot_lights.append(L)
+        return len(self.spot_lights) - 1
+
+    def add_distant_light(self, direction, ke):
+        """Directional light arriving from `direction` (towards the light), which
+        the reference normalises in its constructor (noor/cuda/light.cuh:228-240)."""
+        d = np.asarray(direction, np.float64)
+        d = d / np.linalg.norm(d)
+        L = abi.DistantLight()
+        L.ke = abi.float3(*map(float, ke))
+        L.power = abi.float3(0, 0, 0)
+        L.type = abi.LIGHT_DISTANT
+        L.direction = abi.float3(*map(float, d))
+        self.distant_lights.append(L)
+        return len(self.distant_lights) - 1
+
     def set_env(self, rgba):
         """Environment texture (RGBA32F, H x W x 4), stored as the last texture."""
         self.env = np.ascontiguousarray(rgba, np.float32)
@@ -219,6 +292,12 @@ class SceneBuilder:
         d.textures = tvs
         d.num_area_lights = len(self.area_lights)
         d.area_lights = lights
+        for name, T in (("point", abi.PointLight), ("spot", abi.SpotLight), ("distant", abi.DistantLight)):
+            ls = getattr(self, name + "_lights")
+            arr = (T * max(1, len(ls)))(*ls)
+            keep.append(arr)
+            setattr(d, f"num_{name}_lights", len(ls))
+            setattr(d, f"{name}_lights", arr)
         bp = abi.BvhParams(num_bins, max_height, min_leaf, max_leaf, Ci, Ct, threads)
         h = C.c_void_p()
         check(lib().noor_scene_build(C.byref(d), C.byref(bp), C.byref(h)))

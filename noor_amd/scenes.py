@@ -148,9 +148,14 @@ def fbm2(x, z, octaves, seed, base_freq):
 
 
 def quad(corner, u, v):
+    """Two triangles over corner + [0,1]u + [0,1]v, wound so that the geometric
+    normal (and the vertex normals derived from it) is v x u.  The path's
+    sidedness test (noor/cuda/accumulate.cuh:41-45) ends a path scattered to
+    the side the geometric normal points away from, so walls must face the
+    room."""
     c, u, v = (np.asarray(a, np.float64) for a in (corner, u, v))
     pos = np.stack([c, c + u, c + u + v, c + v])
-    return pos, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    return pos, np.array([[0, 2, 1], [0, 3, 2]], np.uint32)
 
 
 def random_rotation(rng):
@@ -421,6 +426,37 @@ def batch_views(n=48, seed=7, center=(50.0, 45.0, 50.0), sizes=((256, 256), (512
         cam = make_camera(tuple(eye), tuple(look), (0, 1, 0), np.radians(rng.uniform(35.0, 60.0)), w, h)
         out.append((cam, 1, spp, int(rng.integers(1 << 20))))
     return out
+
+
+def lights(seed=6, subdiv=4, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, sky=True,
+           rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL):
+    """Every light type of the reference beside the HDR environment: a
+    spherical and a disk area light, a point, a spot and a distant light, over
+    diffuse walls, a glossy block and a copper sphere in the open Cornell
+    geometry.  With lights present the reference turns the sky off
+    (noor/source/scene.cpp:51-91); `sky` turns it back on so that the infinite
+    light is among the sampled ones."""
+    sb, rng, meshes, (sp, sf), lam = _cornell_builder(seed, subdiv, with_ceiling=False)
+    walls = [lam() for _ in range(4)]
+    block = sb.add_material(material(abi.MAT_GLOSSY, diffuse=(0.3, 0.3, 0.6), specular=(0.4, 0.4, 0.4),
+                                     roughness=(0.2, 0.2)))
+    metal = sb.add_material(material(abi.MAT_METAL, eta=CU_ETA, k=CU_K, roughness=(0.15, 0.15)))
+    for k, (pos, tri) in enumerate(meshes):
+        sb.add_mesh(Mesh(pos, tri, material=walls[k] if k < 4 else block))
+    sb.add_mesh(Mesh(sp, sf, material=metal))
+    sb.add_sphere_light((0.3, 0.55, -0.35), 0.12, (9.0, 8.0, 6.0))
+    sb.add_disk_light((-0.45, 0.9, 0.2), (0.0, -1.0, 0.0), 0.2, (4.0, 4.5, 6.0))
+    sb.add_point_light((0.05, 0.45, 0.35), (0.6, 0.55, 0.45))
+    sb.add_spot_light((0.55, 0.75, 0.55), (-0.35, 1.0, -0.35), 0.96, 0.85, (3.0, 2.2, 1.5))
+    sb.add_distant_light((0.3, 1.0, 0.6), (0.8, 0.75, 0.7))
+    sb.set_env(env_map(128, 64, seed + 3, lobes=2, sigma_deg=8.0, peak=20.0))
+    payload = sb.build()
+    cam = make_camera((0.0, 0.0, 0.95), (0.0, -0.1, -1.0), (0, 1, 0), np.radians(60.0), width, height)
+    spec = make_spec(payload, bounces=bounces, rr=rr, skydome_type=abi.SKY_HDR, mis=mis)
+    if sky:
+        spec.lighting_type |= abi.LIGHT_SKY_MASK
+    opt = make_options(rng=rng_kind, compat=compat)
+    return Workload("lights", payload, cam, spec, opt, None, spp, info=dict(triangles=payload.view.num_triangles))
 
 
 def showcase(seed=5, subdiv=4, width=96, height=80, spp=4, bounces=6, rr=3, mis=False,

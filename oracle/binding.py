@@ -264,3 +264,21 @@ def env_sample(scene, u1, u2):
     q = np.zeros(1, np.float32)
     L.oracle_env_sample(scene._h, u1, u2, _fp(d), _fp(p), _fp(q))
     return d, float(p[0]), float(q[0])
+
+
+def light_sample(scene, idx, p, n, gid, frame=1):
+    """One sample of light `idx` (area, point, spot, distant, infinite in that
+    order) seen from point p with normal n: (Li without the light-count
+    factor, unit direction, distance, pdf, pdf_Li of that direction)."""
+    L = lib()
+    f = C.POINTER(C.c_float)
+    L.oracle_light_sample.argtypes = [C.c_void_p, C.c_int, f, f, C.c_uint32, C.c_uint32, f, f, f, f, f]
+    pp = np.ascontiguousarray(p, np.float32)
+    nn = np.ascontiguousarray(n, np.float32)
+    Li = np.zeros(3, np.float32)
+    wi = np.zeros(3, np.float32)
+    d = np.zeros(1, np.float32)
+    q = np.zeros(1, np.float32)
+    ql = np.zeros(1, np.float32)
+    L.oracle_light_sample(scene._h, idx, _fp(pp), _fp(nn), gid, frame, _fp(Li), _fp(wi), _fp(d), _fp(q), _fp(ql))
+    return Li, wi, float(d[0]), float(q[0]), float(ql[0])

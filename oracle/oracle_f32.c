@@ -111,6 +111,29 @@ void oracle_env_sample(orc_scene* s, float u1, float u2, float* dir, float* pdf,
     *pdf_li = light_pdf_Li(s, &I, d, s->num_lights - 1);
 }
 
+/* One sample of light idx seen from point p with normal n (PCG32 stream gid,
+ * frame): the unscaled Li, the direction and distance to the sampled point,
+ * the sample's pdf and pdf_Li of the same direction. */
+void oracle_light_sample(orc_scene* s, int idx, const float* p, const float* n, uint32_t gid, uint32_t frame,
+                         float* Li, float* wi, float* dist, float* pdf, float* pdf_li) {
+    isect_t I;
+    memset(&I, 0, sizeof(I));
+    I.p = mk3(p[0], p[1], p[2]);
+    I.gn = I.sn = mk3(n[0], n[1], n[2]);
+    tctx c;
+    memset(&c, 0, sizeof(c));
+    rng_init(&c.rng, NOOR_RNG_PCG32, gid, frame);
+    vis_t v;
+    memset(&v, 0, sizeof(v));
+    real q = 0;
+    const v3 L = light_sample_one(s, s, &I, idx, &v, &q, &c);
+    Li[0] = L.x; Li[1] = L.y; Li[2] = L.z;
+    wi[0] = v.wi.x; wi[1] = v.wi.y; wi[2] = v.wi.z;
+    *dist = v.dist;
+    *pdf = q;
+    *pdf_li = (q > 0) ? light_pdf_Li(s, &I, v.wi, idx) : 0.0f;
+}
+
 /* exported portable math for the accuracy tests */
 float oracle_sinf(float x) { return om_sinf(x); }
 float oracle_cosf(float x) { return om_cosf(x); }

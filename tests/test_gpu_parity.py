@@ -168,3 +168,25 @@ def test_secondary_materials_vs_oracle(mis):
         w = Renderer.from_workload(wl)
         w.render_wavefront(spp=4)
         np.testing.assert_array_equal(w.film(), g)
+
+
+@pytest.mark.parametrize("mis,sky,compat", [(False, True, abi.COMPAT_ALL), (True, True, abi.COMPAT_ALL),
+                                             (False, False, abi.COMPAT_ALL), (True, True, 0)])
+def test_all_light_types_vs_oracle(mis, sky, compat):
+    """Sphere and disk area lights, point, spot and distant lights (and the
+    environment when `sky`), with the reference's quirks and with all of them
+    corrected: megakernel against the oracle and, without MIS, the wavefront
+    renderer against the megakernel."""
+    wl = scenes.lights(mis=mis, sky=sky, compat=compat)
+    r = Renderer.from_workload(wl)
+    r.render(spp=4)
+    g = r.film()
+    c, _ = OracleScene.from_workload(wl).render(1, 4)
+    err = mean_rel_err(g, c)
+    print(f"mis={mis} sky={sky} compat={compat}: megakernel vs oracle {err:.3e}, identical {np.mean(np.all(g == c, axis=-1)):.4f}")
+    assert err <= TOL_IMAGE
+    assert float(g[..., :3].mean()) > 0.0
+    if not mis:
+        w = Renderer.from_workload(wl)
+        w.render_wavefront(spp=4)
+        np.testing.assert_array_equal(w.film(), g)
