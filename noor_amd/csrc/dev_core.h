@@ -317,10 +317,11 @@ __device__ __forceinline__ WideNode load_wnode(const DevScene& S, uint32_t idx) 
     return n;
 }
 // Plane c (byte c of q) of an axis with origin org and step s: the decode the
-// builder checked (quantize in wide_bvh.cpp), exact product, one rounding.
+// builder checked (quantize in wide_bvh.cpp).  q * s is exact (8-bit q, s a
+// power of two), so the fused multiply-add rounds org + q * s once, exactly
+// as the builder's separate product and sum do, in one instruction.
 __device__ __forceinline__ float q_plane(float org, uint32_t q, int c, float s) {
-    const float qs = (float)((q >> (8 * c)) & 0xFFu) * s;
-    return org + qs;
+    return __builtin_fmaf((float)((q >> (8 * c)) & 0xFFu), s, org);
 }
 
 // Slab test of the four children of a compressed node.  Bit c of the result
