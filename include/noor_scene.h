@@ -84,6 +84,14 @@ typedef struct noor_payload noor_payload;
 
 noor_status noor_scene_build(const noor_scene_desc* desc, const noor_bvh_params* params,
                              noor_payload** out);
+/* The same build with the bottom-level trees of meshes of 16384 or more
+ * triangles built on HIP device `device` (a level-by-level binned SAH that makes
+ * the host builder's decisions with the same arithmetic: the payload is the
+ * same, byte for byte).  Replaces the single-threaded recursion of
+ * noor/source/bvh.cpp:180-258 and noor/source/splitter.cpp:62-161 for large
+ * meshes; needs num_bins <= 16 (NOOR_ERR_UNSUPPORTED otherwise). */
+noor_status noor_scene_build_device(const noor_scene_desc* desc, const noor_bvh_params* params, int device,
+                                    noor_payload** out);
 noor_status noor_payload_view_get(const noor_payload* p, noor_payload_view* view);
 noor_status noor_payload_stats(const noor_payload* p, noor_build_stats* stats);
 void noor_payload_free(noor_payload* p);

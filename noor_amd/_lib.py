@@ -68,6 +68,7 @@ def lib():
         "noor_wide_bvh_check": (C.c_int, [P(abi.PayloadView), P(C.c_uint32), P(C.c_uint32)]),
         "noor_abi_struct_size": (C.c_size_t, [C.c_char_p]),
         "noor_scene_build": (C.c_int, [P(abi.SceneDesc), P(abi.BvhParams), P(C.c_void_p)]),
+        "noor_scene_build_device": (C.c_int, [P(abi.SceneDesc), P(abi.BvhParams), C.c_int, P(C.c_void_p)]),
         "noor_payload_view_get": (C.c_int, [C.c_void_p, P(abi.PayloadView)]),
         "noor_payload_stats": (C.c_int, [C.c_void_p, P(abi.BuildStats)]),
         "noor_payload_free": (None, [C.c_void_p]),

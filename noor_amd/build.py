@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnoor_hip.so")
 BUILD = os.path.join(ROOT, "build", "noor_amd")
 
-HIP_SOURCES = ["capi.hip", "kernels_mega.hip", "kernels_env.hip", "kernels_wave.hip", "kernels_adjoint.hip"]
+HIP_SOURCES = ["capi.hip", "kernels_mega.hip", "kernels_env.hip", "kernels_wave.hip", "kernels_adjoint.hip",
+               "bvh_build.hip"]
 CXX_SOURCES = ["scene_build.cpp", "wide_bvh.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -15,6 +15,8 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <future>
@@ -26,6 +28,7 @@
 #include <vector>
 
 #include "../../include/noor_scene.h"
+#include "bvh_gpu.h"
 
 namespace {
 
@@ -211,7 +214,8 @@ struct Builder {
             note_leaf(level, count);
             return;
         }
-        uint32_t* mid = std::partition(idx + begin, idx + end, [&](uint32_t i) {
+        // stable, so that the device builder (which partitions stably) produces the same order
+        uint32_t* mid = std::stable_partition(idx + begin, idx + end, [&](uint32_t i) {
             return prims[i].c[s.axis] <= s.bound;
         });
         const size_t split = (size_t)(mid - idx);
@@ -355,8 +359,26 @@ uint32_t stack_need(const std::vector<Node>& nodes, uint32_t ni, std::vector<int
 
 }  // namespace
 
+namespace {
+noor_status build_impl(const noor_scene_desc* d, const noor_bvh_params* bp, int device, noor_payload** out);
+}  // namespace
+
 extern "C" noor_status noor_scene_build(const noor_scene_desc* d, const noor_bvh_params* bp,
                                         noor_payload** out) {
+    return build_impl(d, bp, -1, out);
+}
+
+extern "C" noor_status noor_scene_build_device(const noor_scene_desc* d, const noor_bvh_params* bp, int device,
+                                               noor_payload** out) {
+    if (device < 0) return NOOR_ERR_INVALID_ARG;
+    return build_impl(d, bp, device, out);
+}
+
+namespace {
+// Meshes of at least this many triangles are built on the device.
+constexpr uint32_t kDeviceMinTris = 16384;
+
+noor_status build_impl(const noor_scene_desc* d, const noor_bvh_params* bp, int device, noor_payload** out) {
     if (!d || !out) return NOOR_ERR_INVALID_ARG;
     *out = nullptr;
     const auto t0 = std::chrono::steady_clock::now();
@@ -443,11 +465,20 @@ extern "C" noor_status noor_scene_build(const noor_scene_desc* d, const noor_bvh
             ths.emplace_back([&, th] {
                 for (uint32_t m = 0; m < d->num_meshes; ++m) {
                     const noor_mesh_desc& md = d->meshes[m];
-                    for (uint32_t t = th; t < md.num_triangles; t += T) {
+                    // a contiguous share per thread (no two threads write one cache line)
+                    const uint32_t t0 = (uint32_t)((uint64_t)md.num_triangles * th / T);
+                    const uint32_t t1 = (uint32_t)((uint64_t)md.num_triangles * (th + 1) / T);
+                    for (uint32_t t = t0; t < t1; ++t) {
                         Prim& pr = prims[toff[m] + t];
                         pr.box = Box();
                         for (int k = 0; k < 3; ++k) pr.box.include(&md.positions[3 * (size_t)md.indices[3 * (size_t)t + k]]);
-                        for (int a = 0; a < 3; ++a) pr.c[a] = pr.box.centroid(a);
+                        // -0 becomes +0 (x + 0 = +0 for x = -0): minima and maxima then do
+                        // not depend on the order they are taken in
+                        for (int a = 0; a < 3; ++a) {
+                            pr.box.mn[a] += 0.0f;
+                            pr.box.mx[a] += 0.0f;
+                            pr.c[a] = pr.box.centroid(a) + 0.0f;
+                        }
                         tri_mesh[toff[m] + t] = m;
                     }
                 }
@@ -463,6 +494,7 @@ extern "C" noor_status noor_scene_build(const noor_scene_desc* d, const noor_bvh
     nodes.reserve((size_t)nt * 2 + 1024);
     std::vector<uint32_t> blas_root(d->num_meshes, 0);
     std::vector<Box> mesh_box(d->num_meshes);
+    double device_ms = 0.0;
     Builder B;
     B.prims = prims.data();
     B.idx = perm.data();
@@ -480,6 +512,36 @@ extern "C" noor_status noor_scene_build(const noor_scene_desc* d, const noor_bvh
         if (toff[m + 1] == toff[m]) {
             // empty mesh: a leaf with no triangles
             nodes.push_back(make_node(box, toff[m], NOOR_TRI_LEAF_MASK | 0u));
+            continue;
+        }
+        const uint32_t mt = toff[m + 1] - toff[m];
+        if (device >= 0 && mt >= kDeviceMinTris) {
+            // the mesh's primitives in input order (perm is still the identity here), as the
+            // device builder's record: bounds then centroid, 9 floats
+            static_assert(sizeof(Prim) == sizeof(noor::GpuPrim) && offsetof(Prim, c) == offsetof(noor::GpuPrim, c),
+                          "Prim and GpuPrim share a layout");
+            const noor::GpuPrim* gp = reinterpret_cast<const noor::GpuPrim*>(&prims[toff[m]]);
+            const float rb[6] = {box.mn[0], box.mn[1], box.mn[2], box.mx[0], box.mx[1], box.mx[2]};
+            const float rc[6] = {cent.mn[0], cent.mn[1], cent.mn[2], cent.mx[0], cent.mx[1], cent.mx[2]};
+            const noor::GpuBvhParams gpar{P.num_bins, P.max_height, P.min_leaf, P.max_leaf, P.Ci, P.Ct};
+            noor::GpuBvhResult res;
+            const std::string err =
+                noor::gpu_build_blas(device, gp, mt, rb, rc, gpar, (uint32_t)nodes.size(), toff[m], res);
+            if (!err.empty()) {
+                std::fprintf(stderr, "noor_scene_build_device: %s\n", err.c_str());
+                return err.find("bins") != std::string::npos ? NOOR_ERR_UNSUPPORTED : NOOR_ERR_HIP;
+            }
+            for (uint32_t t = 0; t < mt; ++t) perm[toff[m] + t] = toff[m] + res.perm[t];
+            const size_t nn = res.nodes.size() / 8, base = nodes.size();
+            nodes.resize(base + nn);
+            std::memcpy(&nodes[base], res.nodes.data(), nn * sizeof(Node));
+            uint32_t h = B.height.load();
+            while (h < res.height && !B.height.compare_exchange_weak(h, res.height)) {}
+            uint32_t ml = B.max_leaf.load();
+            while (ml < res.max_leaf && !B.max_leaf.compare_exchange_weak(ml, res.max_leaf)) {}
+            B.leaves += res.leaves;
+            B.inners += res.inners;
+            device_ms += res.device_ms;
             continue;
         }
         B.build_tri(toff[m], toff[m + 1], box, cent, 0, nodes, 0);
@@ -588,9 +650,10 @@ extern "C" noor_status noor_scene_build(const noor_scene_desc* d, const noor_bvh
     // ---- payload arrays ------------------------------------------------------------------------
     p->bvh_a.resize(nodes.size());
     p->bvh_b.resize(nodes.size());
+    static_assert(sizeof(noor_uint4) == 16 && sizeof(Node) == 32, "node words");
     for (size_t i = 0; i < nodes.size(); ++i) {
-        p->bvh_a[i] = {nodes[i].a[0], nodes[i].a[1], nodes[i].a[2], nodes[i].a[3]};
-        p->bvh_b[i] = {nodes[i].b[0], nodes[i].b[1], nodes[i].b[2], nodes[i].b[3]};
+        std::memcpy(&p->bvh_a[i], nodes[i].a, 16);
+        std::memcpy(&p->bvh_b[i], nodes[i].b, 16);
     }
     p->attr.resize(nt);
     p->wald0.resize(nt);
@@ -602,7 +665,8 @@ extern "C" noor_status noor_scene_build(const noor_scene_desc* d, const noor_bvh
         const int T = std::min<int>(threads, std::max<int>(1, (int)(nt / 65536) + 1));
         for (int th = 0; th < T; ++th) {
             ths.emplace_back([&, th] {
-                for (uint32_t pos = th; pos < nt; pos += T) {
+                const uint32_t p0 = (uint32_t)((uint64_t)nt * th / T), p1 = (uint32_t)((uint64_t)nt * (th + 1) / T);
+                for (uint32_t pos = p0; pos < p1; ++pos) {
                     const uint32_t t = perm[pos];
                     const uint32_t m = tri_mesh[t];
                     const noor_mesh_desc& md = d->meshes[m];
@@ -673,9 +737,11 @@ extern "C" noor_status noor_scene_build(const noor_scene_desc* d, const noor_bvh
                 dz = scene_box.mx[2] - scene_box.mn[2];
     st.scene_radius = 0.5f * std::sqrt(dx * dx + dy * dy + dz * dz);
     st.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    (void)device_ms;
     *out = p.release();
     return NOOR_OK;
 }
+}  // namespace
 
 extern "C" noor_status noor_payload_view_get(const noor_payload* p, noor_payload_view* v) {
     if (!p || !v) return NOOR_ERR_INVALID_ARG;

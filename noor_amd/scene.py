@@ -251,7 +251,9 @@ class SceneBuilder:
         """Environment texture (RGBA32F, H x W x 4), stored as the last texture."""
         self.env = np.ascontiguousarray(rgba, np.float32)
 
-    def build(self, num_bins=16, max_height=32, min_leaf=2, max_leaf=2, Ci=1.0, Ct=0.125, threads=0):
+    def build(self, num_bins=16, max_height=32, min_leaf=2, max_leaf=2, Ci=1.0, Ct=0.125, threads=0, device=None):
+        """The payload (noor_scene_build); with `device` (a HIP device index) the
+        trees of large meshes are built on that GPU (noor_scene_build_device)."""
         keep = []
         mds = (abi.MeshDesc * max(1, len(self.meshes)))()
         for i, m in enumerate(self.meshes):
@@ -300,7 +302,10 @@ class SceneBuilder:
             setattr(d, f"{name}_lights", arr)
         bp = abi.BvhParams(num_bins, max_height, min_leaf, max_leaf, Ci, Ct, threads)
         h = C.c_void_p()
-        check(lib().noor_scene_build(C.byref(d), C.byref(bp), C.byref(h)))
+        if device is None:
+            check(lib().noor_scene_build(C.byref(d), C.byref(bp), C.byref(h)))
+        else:
+            check(lib().noor_scene_build_device(C.byref(d), C.byref(bp), int(device), C.byref(h)))
         return Payload(h, keep)
 
 

@@ -280,7 +280,8 @@ def cornell(seed=1, subdiv=6, width=512, height=512, spp=16, bounces=5, rr=3,
 
 
 def procedural(seed=2, width=1920, height=1080, spp=64, bounces=8, rr=3, target_tris=1_000_000,
-               rng_kind=abi.RNG_PCG32, tri_test=abi.TRI_WATERTIGHT, compat=abi.COMPAT_ALL, n_meshes=256):
+               rng_kind=abi.RNG_PCG32, tri_test=abi.TRI_WATERTIGHT, compat=abi.COMPAT_ALL, n_meshes=256,
+               build_device=None):
     rng = np.random.default_rng(seed)
     sb = SceneBuilder()
     n_hf = max(1, int(round(4 * n_meshes / 256)))
@@ -321,7 +322,7 @@ def procedural(seed=2, width=1920, height=1080, spp=64, bounces=8, rr=3, target_
         M[:3, :3] = random_rotation(rng) * rng.uniform(0.5, 3.0)
         M[:3, 3] = rng.uniform(0.0, 100.0, 3)
         sb.add_instance(idx, M)
-    payload = sb.build()
+    payload = sb.build(device=build_device)
     cam = make_camera((50.0, 75.0, 210.0), (50.0, 45.0, 50.0), (0, 1, 0), np.radians(45.0), width, height)
     spec = make_spec(payload, bounces=bounces, rr=rr, skydome_type=abi.SKY_PHYSICAL)
     opt = make_options(rng=rng_kind, tri_test=tri_test, compat=compat)
@@ -331,7 +332,7 @@ def procedural(seed=2, width=1920, height=1080, spp=64, bounces=8, rr=3, target_
 
 def terrain(seed=3, grid=1582, n_spheres=1000, width=1920, height=1080, spp=32, bounces=8, rr=3,
             env_w=2048, env_h=1024, rng_kind=abi.RNG_PCG32, tri_test=abi.TRI_WATERTIGHT,
-            compat=abi.COMPAT_ALL, extent=1000.0):
+            compat=abi.COMPAT_ALL, extent=1000.0, build_device=None):
     rng = np.random.default_rng(seed)
     sb = SceneBuilder()
     ter = sb.add_material(material(abi.MAT_DIFFUSE, diffuse=(0.45, 0.42, 0.38)))
@@ -359,7 +360,7 @@ def terrain(seed=3, grid=1582, n_spheres=1000, width=1920, height=1080, spp=32, 
     sb.add_instance(0, np.eye(4))
     # mesh 0 (terrain) must be instanced explicitly once; sphere meshes only via instances
     sb.set_env(env_map(env_w, env_h, seed + 7))
-    payload = sb.build()
+    payload = sb.build(device=build_device)
     cam = make_camera((0.0, 90.0, 0.48 * extent), (0.0, 0.0, 0.0), (0, 1, 0), np.radians(50.0), width, height)
     spec = make_spec(payload, bounces=bounces, rr=rr, skydome_type=abi.SKY_HDR)
     opt = make_options(rng=rng_kind, tri_test=tri_test, compat=compat)
