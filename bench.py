@@ -221,6 +221,7 @@ def main():
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-build-compare", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
     args = ap.parse_args()
     if args.pmc_probe:
@@ -244,6 +245,23 @@ def main():
     tri_test = wl.options.tri_test
     log(f"scene {args.workload}: {wl.payload.view.num_triangles} triangles, BVH {wl.payload.stats.num_nodes} nodes, "
         f"build {wl.payload.stats.build_seconds:.1f}s, total setup {time.time() - t0:.1f}s")
+    build = None
+    if args.workload in ("terrain", "procedural") and not args.no_build_compare:
+        # the same payload built with the bottom-level trees on the GPU, twice
+        # (the second reuses the builder's device memory); untimed by the steps
+        import numpy as np
+        times, same = [], True
+        for _ in range(2):
+            wd = fn(build_device=0, **kw)
+            times.append(wd.payload.stats.build_seconds)
+            a, b = wl.payload.arrays(), wd.payload.arrays()
+            same = same and all(np.array_equal(a[k], b[k]) for k in a)
+            del wd
+        build = {"triangles": wl.payload.view.num_triangles, "host_ms": round(wl.payload.stats.build_seconds * 1e3, 1),
+                 "device_ms_first": round(times[0] * 1e3, 1), "device_ms_repeat": round(times[1] * 1e3, 1),
+                 "identical_payload": bool(same)}
+        log(f"scene build: host {build['host_ms']} ms, with device trees {build['device_ms_first']} ms "
+            f"(repeat {build['device_ms_repeat']} ms), identical payload {same}")
 
     r = Renderer.from_workload(wl)
     r.set_wavefront_params(args.queue, args.block)
@@ -370,6 +388,7 @@ def main():
         out["cpu_baseline"] = None
     r.close()
     del r
+    out["scene_build"] = build
     if not args.no_batch:
         out["batch"] = bench_batch()
     if not args.no_adjoint:
