@@ -233,6 +233,7 @@ class BuildStats(C.Structure):
 MAT_DIFFUSE, MAT_TRANSLUCENT, MAT_GLOSSY, MAT_MIRROR = 1, 4, 8, 16
 MAT_GLASS, MAT_ROUGHGLASS, MAT_METAL, MAT_EMITTER = 32, 64, 128, 512
 MAT_SUBSTRATE, MAT_CLEARCOAT, MAT_SHADOWCATCHER = 256, 1024, 2048
+MAT_ALPHA, MAT_BUMP = 1 << 12, 1 << 13
 MAT_MESHLIGHT = 1 << 14
 LIGHT_POINT, LIGHT_SPOT, LIGHT_DISTANT, LIGHT_AREA, LIGHT_INFINITE = 1, 2, 4, 8, 16
 SHAPE_QUAD, SHAPE_SPHERE, SHAPE_DISK = 0, 1, 2

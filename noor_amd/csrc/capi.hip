@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "texture_store.h"
 #include "wide_bvh.h"
 #include "noor_scene.h"
 
@@ -89,7 +90,10 @@ uint32_t stack_need(const noor_payload_view* P, uint32_t ni, std::vector<int32_t
 
 const uint32_t kSupportedMaterials = NOOR_MAT_DIFFUSE | NOOR_MAT_TRANSLUCENT | NOOR_MAT_GLOSSY | NOOR_MAT_MIRROR |
                                      NOOR_MAT_GLASS | NOOR_MAT_METAL | NOOR_MAT_EMITTER | NOOR_MAT_ROUGHGLASS |
-                                     NOOR_MAT_SUBSTRATE | NOOR_MAT_CLEARCOAT | NOOR_MAT_SHADOWCATCHER;
+                                     NOOR_MAT_SUBSTRATE | NOOR_MAT_CLEARCOAT | NOOR_MAT_SHADOWCATCHER |
+                                     NOOR_MAT_ALPHA | NOOR_MAT_BUMP;
+
+bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
 noor_status validate(Context* c, const noor_payload_view* P) {
     if (P->num_triangles && (!P->attribute_indices || !P->wald_ax_nu_nv_nd || !P->wald_bnu_bnv_au_av ||
@@ -114,6 +118,9 @@ noor_status validate(Context* c, const noor_payload_view* P) {
         if (a.w >= P->num_materials)
             return fail(c, NOOR_ERR_INVALID_PAYLOAD, "material index out of range in triangle " + std::to_string(t));
         if (P->wald_ax_nu_nv_nd[t].x > 2u) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "bad Wald axis");
+        // the alpha test reads the transparency map of the triangle's material
+        if ((P->wald_cnu_cnv_deg_mat[t].w & NOOR_MAT_ALPHA) && !(P->materials[a.w].type & NOOR_MAT_ALPHA))
+            return fail(c, NOOR_ERR_INVALID_PAYLOAD, "alpha-tested triangle of a material without a transparency map");
     }
     for (uint32_t n = 0; n < P->num_bvh_nodes; ++n) {
         const noor_uint4 a = P->bvh_min_right_start[n];
@@ -140,13 +147,20 @@ noor_status validate(Context* c, const noor_payload_view* P) {
         if ((mt.type & ~kSupportedMaterials) != 0 || mt.type == 0)
             return fail(c, NOOR_ERR_UNSUPPORTED, "material type " + std::to_string(mt.type) + " not supported");
         const uint32_t idx[] = {mt.diffuse_tex_idx, mt.specular_tex_idx, mt.transmission_tex_idx,
-                                mt.emittance_tex_idx, mt.ior_tex_idx, mt.k_tex_idx, mt.roughness_tex_idx};
+                                mt.emittance_tex_idx, mt.ior_tex_idx, mt.k_tex_idx, mt.roughness_tex_idx,
+                                (mt.type & NOOR_MAT_BUMP) ? mt.bump_tex_idx : mt.diffuse_tex_idx,
+                                (mt.type & NOOR_MAT_ALPHA) ? mt.transparency_tex_idx : mt.diffuse_tex_idx};
         for (uint32_t t : idx) {
             if (t >= P->num_textures) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "texture index out of range");
             const noor_texture_view& tv = P->textures[t];
-            if (tv.width != 1 || tv.height != 1 || !tv.rgba)
-                return fail(c, NOOR_ERR_UNSUPPORTED, "only 1x1 material textures are supported");
+            if (!tv.rgba || tv.width == 0 || tv.height == 0)
+                return fail(c, NOOR_ERR_INVALID_PAYLOAD, "empty material texture");
+            if (tv.width > 1u << 15 || tv.height > 1u << 15 || !is_pow2(tv.width) || !is_pow2(tv.height))
+                return fail(c, NOOR_ERR_UNSUPPORTED,
+                            "material textures must be powers of two in each dimension, at most 32768");
         }
+        if ((mt.type & NOOR_MAT_BUMP) && !std::isfinite(mt.bumpfactor))
+            return fail(c, NOOR_ERR_INVALID_PAYLOAD, "non-finite bump factor");
     }
     const noor_texture_view& env = P->textures[P->num_textures - 1];
     if (!env.rgba || env.width == 0 || env.height == 0) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "no environment texture");
@@ -383,9 +397,52 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
             o[1] = px[1];
             o[2] = px[2];
         };
+        // image textures (and every bump or transparency map) go to the texture
+        // store with their mip chains; 1x1 textures of the other fields are
+        // constants, read here once
+        noor::TextureStore store;
+        std::vector<int64_t> store_idx(P->num_textures, -1);
+        auto stored = [&](uint32_t t) -> uint32_t {
+            if (store_idx[t] < 0) {
+                const noor_texture_view& tv = P->textures[t];
+                store_idx[t] = noor::store_add_texture(store, tv.rgba, tv.width, tv.height);
+            }
+            return (uint32_t)store_idx[t];
+        };
+        S.textured = 0;
         for (uint32_t m = 0; m < P->num_materials; ++m) {
             const noor_material& mt = P->materials[m];
             noor::DevMat& d = mats[m];
+            std::memset(&d, 0, sizeof(d));
+            const noor_float2 one{1.0f, 1.0f};
+            const struct {
+                uint32_t field, tex;
+                noor_float2 uvs;
+                bool always;
+            } fields[] = {
+                {noor::TX_DIFFUSE, mt.diffuse_tex_idx, mt.diffuse_uvscale, false},
+                {noor::TX_SPECULAR, mt.specular_tex_idx, mt.specular_uvscale, false},
+                {noor::TX_TRANSMISSION, mt.transmission_tex_idx, one, false},
+                {noor::TX_EMITTANCE, mt.emittance_tex_idx, mt.emittance_uvscale, false},
+                {noor::TX_ETA, mt.ior_tex_idx, one, false},
+                {noor::TX_K, mt.k_tex_idx, one, false},
+                {noor::TX_ROUGH, mt.roughness_tex_idx, mt.roughness_uvscale, false},
+                {noor::TX_BUMP, mt.bump_tex_idx, mt.bump_uvscale, true},
+                {noor::TX_ALPHA, mt.transparency_tex_idx, mt.transparency_uvscale, true},
+            };
+            for (const auto& f : fields) {
+                if (f.field == noor::TX_BUMP && !(mt.type & NOOR_MAT_BUMP)) continue;
+                if (f.field == noor::TX_ALPHA && !(mt.type & NOOR_MAT_ALPHA)) continue;
+                const noor_texture_view& tv = P->textures[f.tex];
+                if (!f.always && tv.width * tv.height == 1) continue;
+                d.tex_mask |= 1u << f.field;
+                d.tex[f.field] = stored(f.tex);
+                d.uvs[f.field][0] = f.uvs.x;
+                d.uvs[f.field][1] = f.uvs.y;
+            }
+            d.bumpfactor = mt.bumpfactor;
+            d.emitter_scale = mt.emitter_scale;
+            if (d.tex_mask) S.textured = 1;
             float tmp[3];
             rgb(mt.diffuse_tex_idx, d.diffuse);
             rgb(mt.specular_tex_idx, d.specular);
@@ -404,6 +461,14 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
             d.type = mt.type;
         }
         if ((st = dev_upload(c, (noor::DevMat**)&S.mats, mats.data(), mats.size())) != NOOR_OK) return bail(st);
+        static_assert(sizeof(noor::TexLevels) == sizeof(noor::TexDesc), "texture descriptor layouts differ");
+        if ((st = dev_upload(c, (float4**)&S.tex_texels, (const float4*)store.texels.data(), store.texels.size() / 4)) !=
+            NOOR_OK)
+            return bail(st);
+        if ((st = dev_upload(c, (noor::TexDesc**)&S.tex_desc, (const noor::TexDesc*)store.desc.data(),
+                             store.desc.size())) != NOOR_OK)
+            return bail(st);
+        c->texels = store.texels.size() / 4;
     }
     if ((st = dev_upload(c, (noor_area_light**)&S.area, P->area_lights, P->num_area_lights)) != NOOR_OK) return bail(st);
     if ((st = dev_upload(c, (noor_point_light**)&S.point, P->point_lights, P->num_point_lights)) != NOOR_OK) return bail(st);
@@ -729,6 +794,7 @@ noor_status noor_adjoint(noor_ctx* ctx, uint32_t stream, const float* radiance, 
     if (num_materials && (!grad_albedo || !grad_roughness)) return NOOR_ERR_INVALID_ARG;
     if (num_materials > c->num_materials) return fail(c, NOOR_ERR_INVALID_ARG, "num_materials");
     if (c->S.mis) return fail(c, NOOR_ERR_UNSUPPORTED, "the adjoint does not implement MIS");
+    if (c->S.textured) return fail(c, NOOR_ERR_UNSUPPORTED, "the adjoint does not implement image textures");
     if (c->S.compat & NOOR_COMPAT_CLAMP) return fail(c, NOOR_ERR_UNSUPPORTED, "the adjoint needs the clamp off");
     HIP_TRY(c, noor::launch_adjoint(*c, stream, radiance, grad_image, grad_albedo, grad_roughness, num_materials,
                                     grad_env));

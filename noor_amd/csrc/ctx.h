@@ -102,6 +102,7 @@ struct Context {
     uint32_t stack_need = 0;      // traversal stack entries the wide BVH can need
     int stack_lds = 24;           // of which at most this many per lane live in LDS
     size_t wide_nodes = 0;
+    size_t texels = 0;      // texels of the material texture store, all mip levels
     uint32_t bvh_root = 0;        // the payload's binary root (the spec must name it)
     uint32_t num_materials = 0;
     uint32_t* gstack = nullptr;   // device part of the traversal stack

@@ -64,6 +64,9 @@ enum : int { L_SHADOWCATCHER = 0, L_LAMBERT_R = 1, L_SPEC_R_NOOP = 2, L_MICRO_R_
 constexpr uint32_t M_EMITTER_ANY = NOOR_MAT_EMITTER | NOOR_MAT_MESHLIGHT;
 constexpr uint32_t M_TRANSPARENT = NOOR_MAT_TRANSLUCENT | NOOR_MAT_GLASS | NOOR_MAT_ROUGHGLASS;
 constexpr uint32_t M_SPECULAR = NOOR_MAT_MIRROR | NOOR_MAT_GLASS | NOOR_MAT_CLEARCOAT;
+// materials whose spawned rays carry differentials (noor/cuda/path_tracer.cuh:125-126)
+constexpr uint32_t M_GLOSSY = M_SPECULAR | NOOR_MAT_ROUGHGLASS | NOOR_MAT_CLEARCOAT | NOOR_MAT_METAL |
+                              NOOR_MAT_SUBSTRATE | NOOR_MAT_GLOSSY;
 
 // ---------------------------------------------------------------------------------------------
 // vectors (CUDA-samples helper_math semantics, IEEE ops, no contraction)
@@ -113,12 +116,29 @@ __device__ __forceinline__ f3 fromf4(float4 a) { return f3{a.x, a.y, a.z}; }
 // scene layout in device memory
 
 // 1x1 material textures resolved at upload (noor/cuda/materialimp.cuh:134-214).
+// Material fields that can read an image texture (noor/cuda/materialimp.cuh:120-186)
+enum : uint32_t { TX_DIFFUSE = 0, TX_SPECULAR, TX_TRANSMISSION, TX_EMITTANCE, TX_ETA, TX_K, TX_ROUGH, TX_BUMP,
+                  TX_ALPHA, TX_COUNT };
+
 struct DevMat {
     float diffuse[3], specular[3], transmission[3], emittance[3], eta[3], k[3];
     float rough[2];
     float coat_ior;
     float coat_weight, coat_thickness, coat_sigma;
     uint32_t type;
+    // image textures: bit f of tex_mask set when field f reads texture tex[f]
+    // at uvscale uvs[f] (the constants above are then unused)
+    uint32_t tex_mask;
+    uint32_t tex[TX_COUNT];
+    float uvs[TX_COUNT][2];
+    float bumpfactor, emitter_scale;
+};
+
+// A texture's mip chain in the texel store: level l is max(1, w >> l) by
+// max(1, h >> l) RGBA texels, the levels packed from `offset` (w, h powers
+// of two).
+struct TexDesc {
+    uint32_t offset, w, h, levels;
 };
 
 struct DevScene {
@@ -138,6 +158,9 @@ struct DevScene {
     const float2* __restrict__ uv;
     const float4* __restrict__ xf;       // 9 rows per instance: w2o, o2w, normal
     const DevMat* __restrict__ mats;
+    const float4* __restrict__ tex_texels;  // mip chains of the material textures
+    const TexDesc* __restrict__ tex_desc;   // per texture of the store
+    uint32_t textured;                      // any image texture, bump or alpha material
     const noor_area_light* __restrict__ area;
     const noor_point_light* __restrict__ point;
     const noor_spot_light* __restrict__ spot;
@@ -425,12 +448,17 @@ __device__ __forceinline__ bool tri_watertight(const DevScene& S, const Ray& r, 
     return true;
 }
 
-template <bool STATS>
+__device__ __forceinline__ bool alpha_pass(const DevScene& S, uint32_t tri, float u, float v);
+
+// ALPHA: triangles of alpha-tested materials whose transparency texture reads
+// <= 1e-4 at the hit are not hit (noor/cuda/triangle.cuh:124-135)
+template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool tri_test(const DevScene& S, const Ray& r, uint32_t i, float& t, float& u,
                                          float& v, Counters& c) {
     if (STATS) c.tris++;
-    if (S.tri_test == NOOR_TRI_WALD) return tri_wald(S, r, i, t, u, v, c);
-    return tri_watertight(S, r, i, t, u, v);
+    const bool hit = (S.tri_test == NOOR_TRI_WALD) ? tri_wald(S, r, i, t, u, v, c) : tri_watertight(S, r, i, t, u, v);
+    if (ALPHA && hit && (ldg(S.wald2 + i).w & NOOR_MAT_ALPHA)) return alpha_pass(S, i, u, v);
+    return hit;
 }
 
 // analytic light shapes (noor/cuda/geometry.cuh:106-211)
@@ -586,7 +614,7 @@ __device__ __forceinline__ void closest_begin(const DevScene& S, const Ray& r, C
     t.h.u = t.h.v = 0.0f;
 }
 
-template <bool STATS>
+template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t, const TravStack& st, Counters& c,
                                               bool& overflow, int max_steps) {
     for (int step = 0; step < max_steps; ++step) {
@@ -627,7 +655,7 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
             const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
             for (uint32_t ti = start; ti < end; ++ti) {
                 float tt, u, v;
-                if (!tri_test<STATS>(S, t.lray, ti, tt, u, v, c)) continue;
+                if (!tri_test<STATS, ALPHA>(S, t.lray, ti, tt, u, v, c)) continue;
                 const uint32_t key = ldg(S.prim_ids + ti);
                 bool take = false;
                 if (tt < t.lray.tmax) take = true;
@@ -676,12 +704,12 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
     return false;
 }
 
-template <bool STATS>
+template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool bvh_closest(const DevScene& S, Ray& ray, Hit& h, const TravStack& st, Counters& c,
                                             bool& overflow) {
     ClosestTrav t;
     closest_begin(S, ray, t);
-    while (!closest_steps<STATS>(S, t, st, c, overflow, 1 << 30)) {
+    while (!closest_steps<STATS, ALPHA>(S, t, st, c, overflow, 1 << 30)) {
     }
     ray.tmax = t.lray.tmax;
     h = t.h;
@@ -708,7 +736,7 @@ __device__ __forceinline__ void any_begin(const DevScene& S, const Ray& r, AnyTr
     t.occluded = false;
 }
 
-template <bool STATS>
+template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const TravStack& st, Counters& c,
                                           bool& overflow, int max_steps) {
     for (int step = 0; step < max_steps; ++step) {
@@ -743,7 +771,7 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
             const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
             for (uint32_t ti = start; ti < end; ++ti) {
                 float tt, u, v;
-                if (tri_test<STATS>(S, t.lray, ti, tt, u, v, c)) {
+                if (tri_test<STATS, ALPHA>(S, t.lray, ti, tt, u, v, c)) {
                     t.occluded = true;
                     return true;
                 }
@@ -769,12 +797,12 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
     return false;
 }
 
-template <bool STATS>
+template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool bvh_any(const DevScene& S, const Ray& ray, const TravStack& st, Counters& c,
                                         bool& overflow) {
     AnyTrav t;
     any_begin(S, ray, t);
-    while (!any_steps<STATS>(S, t, st, c, overflow, 1 << 30)) {
+    while (!any_steps<STATS, ALPHA>(S, t, st, c, overflow, 1 << 30)) {
     }
     return t.occluded;
 }
@@ -897,6 +925,129 @@ __device__ __forceinline__ f3 tex_bilinear(const float4* tex, int w, int h, floa
     return r;
 }
 
+// ---- material image textures (noor/cuda/texture.cuh:30-423) -------------------------------
+// Software filtering restated identically in the oracle: bilinear within a
+// level (texel centres at half-integers, wrap addressing) and linear between
+// the two levels around lod = log2(footprint in texels), the footprint being
+// the longer of the two screen-space uv gradients.  The reference samples
+// through CUDA texture units with 64x anisotropy, whose weights are not
+// specified; its images are not reproduced bit for bit here.
+__device__ __forceinline__ int tex_wrap(int x, int n) {
+    const int r = x % n;
+    return r < 0 ? r + n : r;
+}
+__device__ __forceinline__ float4 tex_level(const DevScene& S, const TexDesc& t, uint32_t l, float s, float q) {
+    uint32_t off = t.offset, lw = t.w, lh = t.h;
+    for (uint32_t j = 0; j < l; ++j) {
+        off += lw * lh;
+        lw = lw > 1u ? lw >> 1 : 1u;
+        lh = lh > 1u ? lh >> 1 : 1u;
+    }
+    if (!(fabsf(s) < 1e30f)) s = 0.0f;  // NaN and infinities
+    if (!(fabsf(q) < 1e30f)) q = 0.0f;
+    s = s - floorf(s);  // wrap into [0, 1)
+    q = q - floorf(q);
+    const float x = s * (float)lw - 0.5f, y = q * (float)lh - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float ax = x - fx, ay = y - fy;
+    const int x0 = tex_wrap((int)fx, (int)lw), x1 = tex_wrap((int)fx + 1, (int)lw);
+    const int y0 = tex_wrap((int)fy, (int)lh), y1 = tex_wrap((int)fy + 1, (int)lh);
+    const float4* T = S.tex_texels + off;
+    const float4 t00 = ldg(T + (size_t)y0 * lw + x0), t10 = ldg(T + (size_t)y0 * lw + x1);
+    const float4 t01 = ldg(T + (size_t)y1 * lw + x0), t11 = ldg(T + (size_t)y1 * lw + x1);
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    float4 r;
+    r.x = by * (bx * t00.x + ax * t10.x) + ay * (bx * t01.x + ax * t11.x);
+    r.y = by * (bx * t00.y + ax * t10.y) + ay * (bx * t01.y + ax * t11.y);
+    r.z = by * (bx * t00.z + ax * t10.z) + ay * (bx * t01.z + ax * t11.z);
+    r.w = by * (bx * t00.w + ax * t10.w) + ay * (bx * t01.w + ax * t11.w);
+    return r;
+}
+// tex2D: level 0
+__device__ __forceinline__ float4 tex_eval(const DevScene& S, uint32_t ti, float s, float q) {
+    const TexDesc t = S.tex_desc[ti];
+    return tex_level(S, t, 0, s, q);
+}
+// tex2DGrad: (gx0, gx1) and (gy0, gy1) the uv changes per pixel step in x and y
+__device__ __forceinline__ float4 tex_grad(const DevScene& S, uint32_t ti, float s, float q, float gx0, float gx1,
+                                           float gy0, float gy1) {
+    const TexDesc t = S.tex_desc[ti];
+    const float fw = (float)t.w, fh = (float)t.h;
+    const float ax = gx0 * fw, ay = gx1 * fh, bx = gy0 * fw, by = gy1 * fh;
+    const float width = rmax(sqrtf(ax * ax + ay * ay), sqrtf(bx * bx + by * by));
+    if (!(width > 1.0f) || t.levels <= 1u) return tex_level(S, t, 0, s, q);
+    const float lod = nm_logf(width) * 1.44269504088896340736f;  // log2
+    if (lod >= (float)(t.levels - 1u)) return tex_level(S, t, t.levels - 1u, s, q);
+    const float fl = floorf(lod);
+    const uint32_t l0 = (uint32_t)fl;
+    const float f = lod - fl;
+    const float4 a = tex_level(S, t, l0, s, q), b = tex_level(S, t, l0 + 1u, s, q);
+    const float g = 1.0f - f;
+    return make_float4(g * a.x + f * b.x, g * a.y + f * b.y, g * a.z + f * b.z, g * a.w + f * b.w);
+}
+
+// tex2D with point filtering (the transparency maps, noor/source/model.cpp:286-297):
+// the texel under (s, q) at level 0, wrap addressing
+__device__ __forceinline__ float4 tex_point(const DevScene& S, uint32_t ti, float s, float q) {
+    const TexDesc t = S.tex_desc[ti];
+    if (!(fabsf(s) < 1e30f)) s = 0.0f;
+    if (!(fabsf(q) < 1e30f)) q = 0.0f;
+    s = s - floorf(s);
+    q = q - floorf(q);
+    const int x = tex_wrap((int)floorf(s * (float)t.w), (int)t.w);
+    const int y = tex_wrap((int)floorf(q * (float)t.h), (int)t.h);
+    return ldg(S.tex_texels + t.offset + (size_t)y * t.w + x);
+}
+
+// MaterialManager::getAlpha at the hit's interpolated uv (noor/cuda/materialimp.cuh:159-163)
+__device__ __forceinline__ bool alpha_pass(const DevScene& S, uint32_t tri, float u, float v) {
+    const uint4 ai = ldg(S.attr + tri);
+    const float2 q0 = ldg(S.uv + ai.x), q1 = ldg(S.uv + ai.y), q2 = ldg(S.uv + ai.z);
+    const float w = 1.0f - u - v;
+    const float tu = w * q0.x + u * q1.x + v * q2.x;
+    const float tv = w * (1.0f - q0.y) + u * (1.0f - q1.y) + v * (1.0f - q2.y);
+    const DevMat* m = S.mats + ai.w;
+    const float a = tex_point(S, m->tex[TX_ALPHA], m->uvs[TX_ALPHA][0] * tu, m->uvs[TX_ALPHA][1] * tv).x;
+    return !(a <= 0.0001f);
+}
+
+// Screen-space uv derivatives of a hit (zero for a ray without differentials)
+struct UvGrad {
+    float dudx, dvdx, dudy, dvdy;
+};
+
+// MaterialManager getters: the material with its image textures looked up at
+// (u, v), written to `out` (noor/cuda/materialimp.cuh:120-186,
+// noor/cuda/texture.cuh:114-138: the uv scaled component-wise, the x and y
+// gradients scaled by uvscale.x and uvscale.y).
+__device__ __noinline__ void resolve_material(const DevScene& S, const DevMat* base, float u, float v, UvGrad g,
+                                              DevMat* out) {
+    DevMat m = *base;
+    const uint32_t mask = m.tex_mask;
+    for (uint32_t f = 0; f < TX_COUNT; ++f) {
+        if (!(mask & (1u << f)) || f == TX_BUMP || f == TX_ALPHA) continue;
+        const float sx = m.uvs[f][0], sy = m.uvs[f][1];
+        const float4 c = (f == TX_ROUGH) ? tex_eval(S, m.tex[f], sx * u, sy * v)
+                                         : tex_grad(S, m.tex[f], sx * u, sy * v, sx * g.dudx, sx * g.dvdx,
+                                                    sy * g.dudy, sy * g.dvdy);
+        switch (f) {
+            case TX_DIFFUSE: m.diffuse[0] = c.x; m.diffuse[1] = c.y; m.diffuse[2] = c.z; break;
+            case TX_SPECULAR: m.specular[0] = c.x; m.specular[1] = c.y; m.specular[2] = c.z; break;
+            case TX_TRANSMISSION: m.transmission[0] = c.x; m.transmission[1] = c.y; m.transmission[2] = c.z; break;
+            case TX_EMITTANCE:
+                m.emittance[0] = m.emitter_scale * c.x;
+                m.emittance[1] = m.emitter_scale * c.y;
+                m.emittance[2] = m.emitter_scale * c.z;
+                break;
+            case TX_ETA: m.eta[0] = c.x; m.eta[1] = c.y; m.eta[2] = c.z; break;
+            case TX_K: m.k[0] = c.x; m.k[1] = c.y; m.k[2] = c.z; break;
+            case TX_ROUGH: m.rough[0] = c.x; m.rough[1] = c.y; break;
+            default: break;
+        }
+    }
+    *out = m;
+}
+
 // Hosek-Wilkie (noor/cuda/hosek.cuh:55-101)
 __device__ __forceinline__ float hosek_rad(const noor_hosek_state& st, float theta, float gamma, int ch) {
     const float* c = st.configs[ch];
@@ -960,7 +1111,70 @@ __device__ __forceinline__ bool is_specular(const Isect& I) { return (I.type & M
 __device__ __forceinline__ bool is_transparent(const Isect& I) { return (I.type & M_TRANSPARENT) != 0; }
 __device__ __forceinline__ bool is_shadowcatcher(const Isect& I) { return (I.type & NOOR_MAT_SHADOWCATCHER) != 0; }
 
-__device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, const Hit& h, Isect& I) {
+// Ray differentials (noor/cuda/ray.cuh:26-207): the auxiliary rays one pixel
+// to the right (x) and one pixel down (y).  Carried only by camera rays and
+// the spawns of glossy materials.
+struct RayDiff {
+    f3 ox, oy, dx, dy;
+    int has;
+};
+// Derivatives at a hit (noor/cuda/intersection.cuh:370-440); zero when the
+// ray had no differentials.
+struct HitDiff {
+    f3 dpdx, dpdy, dndu, dndv;
+    UvGrad g;
+};
+
+// NOOR::solve2x2 (noor/cuda/math.cuh:193-207)
+__device__ __forceinline__ void solve2x2(float a00, float a01, float a10, float a11, float b0, float b1, float& x0,
+                                         float& x1) {
+    const float det = a00 * a11 - a10 * a01;
+    if (fabsf(det) < 1e-10f) {
+        x0 = x1 = 0.0f;
+        return;
+    }
+    const float inv = 1.0f / det;
+    x0 = (a11 * b0 - a01 * b1) * inv;
+    x1 = (a00 * b1 - a10 * b0) * inv;
+    if (x0 != x0 || x1 != x1) x0 = x1 = 0.0f;
+}
+
+// CudaIntersection::computeDifferential (noor/cuda/intersection.cuh:399-439)
+__device__ __forceinline__ void hit_differential(const Isect& I, const RayDiff& rd, f3 dpdu, f3 dpdv, HitDiff& hd) {
+    const float d = dot3(I.gn, I.p);
+    const float tx = -(dot3(I.gn, rd.ox) - d) / dot3(I.gn, rd.dx);
+    if (isinf(tx) || tx != tx) return;
+    const f3 px = add3(rd.ox, scl3(rd.dx, tx));
+    const float ty = -(dot3(I.gn, rd.oy) - d) / dot3(I.gn, rd.dy);
+    if (isinf(ty) || ty != ty) return;
+    const f3 py = add3(rd.oy, scl3(rd.dy, ty));
+    hd.dpdx = sub3(px, I.p);
+    hd.dpdy = sub3(py, I.p);
+    int dx, dy;
+    const f3 n = I.gn;
+    if (fabsf(n.x) > fabsf(n.y) && fabsf(n.x) > fabsf(n.z)) {
+        dx = 1;
+        dy = 2;
+    } else if (fabsf(n.y) > fabsf(n.z)) {
+        dx = 0;
+        dy = 2;
+    } else {
+        dx = 0;
+        dy = 1;
+    }
+    solve2x2(c3(dpdu, dx), c3(dpdv, dx), c3(dpdu, dy), c3(dpdv, dy), c3(hd.dpdx, dx), c3(hd.dpdx, dy), hd.g.dudx,
+             hd.g.dvdx);
+    solve2x2(c3(dpdu, dx), c3(dpdv, dx), c3(dpdu, dy), c3(dpdv, dy), c3(hd.dpdy, dx), c3(hd.dpdy, dy), hd.g.dudy,
+             hd.g.dvdy);
+}
+
+template <bool TEX = false>
+__device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, const Hit& h, Isect& I,
+                                           const RayDiff* rd = nullptr, HitDiff* hd = nullptr) {
+    if (TEX) {
+        hd->dpdx = hd->dpdy = hd->dndu = hd->dndv = f3{0.0f, 0.0f, 0.0f};
+        hd->g = UvGrad{0.0f, 0.0f, 0.0f, 0.0f};
+    }
     I.p = ray_at(ray, ray.tmax);
     I.wo = nrm3(scl3(ray.d, -1.0f));
     I.ins = h.inst;
@@ -977,7 +1191,15 @@ __device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, co
     I.type = ldg(S.wald2 + h.prim).w;
     const uint4 ai = ldg(S.attr + I.tri);
     I.mat = ai.w;
-    if (I.type & M_EMITTER_ANY) return;
+    if (I.type & M_EMITTER_ANY) {
+        // an emitter keeps the barycentrics as its uv (computeTangentSpace
+        // returns before interpolating, noor/cuda/intersection.cuh:326-327)
+        if (TEX) {
+            I.u = h.u;
+            I.v = h.v;
+        }
+        return;
+    }
     const float2 q0 = ldg(S.uv + ai.x), q1 = ldg(S.uv + ai.y), q2 = ldg(S.uv + ai.z);
     const f2 uv0 = f2{q0.x, 1.0f - q0.y}, uv1 = f2{q1.x, 1.0f - q1.y}, uv2 = f2{q2.x, 1.0f - q2.y};
     const f3 p0 = fromf4(ldg(S.vert + ai.x)), p1 = fromf4(ldg(S.vert + ai.y)), p2 = fromf4(ldg(S.vert + ai.z));
@@ -1008,9 +1230,19 @@ __device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, co
         dpdv = add3(scl3(dp01, a10), scl3(dp02, a11));
         const f3 cr = cross3(dpdu, dpdv);
         if (dot3(cr, cr) == 0.0f) coordinate_system(I.gn, dpdu, dpdv);
+        if (TEX && rd->has) {
+            const f3 dn01 = sub3(n1, n0), dn02 = sub3(n2, n0);
+            hd->dndu = add3(scl3(dn01, a00), scl3(dn02, a01));
+            hd->dndv = add3(scl3(dn01, a10), scl3(dn02, a11));
+        }
     } else {
         coordinate_system(I.gn, dpdu, dpdv);
+        if (TEX && rd->has) {
+            const f3 dn = nrm3s(nrm3(xf_vec(T, cross3(sub3(n1, n0), sub3(n2, n0)))));
+            if (dot3(dn, dn) != 0.0f) coordinate_system(dn, hd->dndu, hd->dndv);
+        }
     }
+    if (TEX && rd->has) hit_differential(I, *rd, dpdu, dpdv, *hd);
     const f3 n = I.sn;
     const f3 ss = nrm3s(sub3(dpdu, scl3(n, dot3(n, dpdu))));
     const float sign = dot3(cross3(n, ss), dpdv) < 0.0f ? -1.0f : 1.0f;
@@ -1018,6 +1250,73 @@ __device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, co
     I.ts = nrm3s(scl3(cross3(n, ss), sign));
     // CudaShadowCatcher::Eval reads the sky behind the surface (noor/cuda/bxdf.cuh:210-216)
     if (I.type & NOOR_MAT_SHADOWCATCHER) I.catcher = sky_eval(S, scl3(I.wo, -1.0f));
+}
+
+// bump (noor/cuda/accumulate.cuh:47-81): finite differences of the
+// displacement texture over half the uv footprint of a pixel perturb the
+// shading frame
+__device__ __forceinline__ float bump_tex(const DevScene& S, const DevMat& m, float u, float v, const UvGrad& g) {
+    const float sx = m.uvs[TX_BUMP][0], sy = m.uvs[TX_BUMP][1];
+    return tex_grad(S, m.tex[TX_BUMP], sx * u, sy * v, sx * g.dudx, sx * g.dvdx, sy * g.dudy, sy * g.dvdy).x;
+}
+__device__ __noinline__ void bump_frame(const DevScene& S, const DevMat& m, const HitDiff& hd, Isect& I) {
+    float du = 0.5f * (fabsf(hd.g.dudx) + fabsf(hd.g.dudy));
+    float dv = 0.5f * (fabsf(hd.g.dvdx) + fabsf(hd.g.dvdy));
+    du = du == 0.0f ? 0.001f : du;
+    dv = dv == 0.0f ? 0.001f : dv;
+    const float disp = bump_tex(S, m, I.u, I.v, hd.g);
+    const float ud = m.bumpfactor * (bump_tex(S, m, I.u + du, I.v, hd.g) - disp) / du;
+    const float vd = m.bumpfactor * (bump_tex(S, m, I.u, I.v + dv, hd.g) - disp) / dv;
+    const f3 dpdu = add3(I.ss, scl3(I.sn, ud));
+    const f3 dpdv = add3(I.ts, scl3(I.sn, vd));
+    const f3 n = nrm3s(cross3(dpdu, dpdv));
+    I.ss = nrm3s(sub3(dpdu, scl3(n, dot3(n, dpdu))));
+    I.ts = nrm3s(cross3(n, I.ss));
+    I.sn = dot3(I.gn, n) < 0.0f ? scl3(n, -1.0f) : n;
+}
+
+// spawnRay with differentials (noor/cuda/intersection.cuh:115-182): the
+// reflected or refracted auxiliary rays of a glossy material, their spread
+// scaled by 1/4.  `rd` holds the incoming ray's differentials and receives
+// the spawned ray's.  The refraction branch's dndy pairs dndv with dudy, as
+// the reference does.
+__device__ __forceinline__ void spawn_diff(const DevScene& S, const Isect& I, const HitDiff& hd, f3 origin,
+                                           RayDiff& rd) {
+    const f3 wi = I.wi, wo = I.wo, n = I.sn;
+    const f3 ox = add3(origin, hd.dpdx), oy = add3(origin, hd.dpdy);
+    const f3 dwodx = scl3(add3(rd.dx, wo), -1.0f);
+    const f3 dwody = scl3(add3(rd.dy, wo), -1.0f);
+    f3 dx, dy;
+    if (dot3(wi, I.gn) >= 0.0f) {
+        const f3 dndx = add3(scl3(hd.dndu, hd.g.dudx), scl3(hd.dndv, hd.g.dvdx));
+        const f3 dndy = add3(scl3(hd.dndu, hd.g.dudy), scl3(hd.dndv, hd.g.dvdy));
+        const float dDNdx = dot3(dwodx, n) + dot3(wo, dndx);
+        const float dDNdy = dot3(dwody, n) + dot3(wo, dndy);
+        const float wn = dot3(wo, n);
+        dx = add3(sub3(wi, dwodx), scl3(add3(scl3(dndx, wn), scl3(n, dDNdx)), 2.0f));
+        dy = add3(sub3(wi, dwody), scl3(add3(scl3(dndy, wn), scl3(n, dDNdy)), 2.0f));
+    } else {
+        const f3 w = scl3(wo, -1.0f);
+        const float ieta = (I.type & (NOOR_MAT_GLASS | NOOR_MAT_ROUGHGLASS)) ? S.mats[I.mat].coat_ior : 1.0f;
+        const float eta = dot3(wo, n) > 0.0f ? 1.0f / ieta : ieta;
+        const f3 dndx = add3(scl3(hd.dndu, hd.g.dudx), scl3(hd.dndv, hd.g.dvdx));
+        const f3 dndy = add3(scl3(hd.dndv, hd.g.dudy), scl3(hd.dndv, hd.g.dvdy));
+        const float dDNdx = dot3(dwodx, n) + dot3(wo, dndx);
+        const float dDNdy = dot3(dwody, n) + dot3(wo, dndy);
+        const float mu = eta * dot3(w, n) - dot3(wi, n);
+        const float k = eta - (eta * eta * dot3(w, n)) / dot3(wi, n);
+        const float dmudx = k * dDNdx;
+        const float dmudy = k * dDNdy;
+        dx = sub3(add3(wi, scl3(dwodx, eta)), add3(scl3(dndx, mu), scl3(n, dmudx)));
+        dy = sub3(add3(wi, scl3(dwody, eta)), add3(scl3(dndy, mu), scl3(n, dmudy)));
+    }
+    // CudaRay::scaleDifferentials(0.25)
+    const float s = 0.25f, r = 1.0f - s;
+    rd.ox = add3(scl3(origin, r), scl3(ox, s));
+    rd.oy = add3(scl3(origin, r), scl3(oy, s));
+    rd.dx = add3(scl3(wi, r), scl3(dx, s));
+    rd.dy = add3(scl3(wi, r), scl3(dy, s));
+    rd.has = 1;
 }
 
 __device__ __forceinline__ f3 to_local(const Isect& I, f3 v) { return f3{dot3(v, I.ss), dot3(v, I.ts), dot3(v, I.sn)}; }
@@ -1813,21 +2112,43 @@ __device__ __forceinline__ Ray shadow_ray_vis(const DevScene& S, const Isect& I,
 }
 
 // ---------------------------------------------------------------------------------------------
-// camera rays (noor/cuda/cameraimp.cuh:58-269)
-__device__ __forceinline__ Ray gen_ray(const noor_camera& cam, uint32_t x, uint32_t y, Rng& rng, int rk) {
+// camera rays (noor/cuda/cameraimp.cuh:58-269); with TEX also the ray
+// differentials of the neighbouring pixels, which CudaRay::transform carries
+// to world space (origins as points, directions as unnormalised vectors,
+// noor/cuda/ray.cuh:162-175)
+__device__ __forceinline__ f3 env_cam_dir(const noor_camera& cam, float x, float y) {
+    const float theta = K_PI * y / (float)cam.h + K_PI;
+    const float phi = K_2PI * x / (float)cam.w - K_PI_OVER_2;
+    const float sT = nm_sinf(theta), cT = nm_cosf(theta);
+    return nrm3(f3{sT * nm_cosf(phi), cT, sT * nm_sinf(phi)});
+}
+
+template <bool TEX = false>
+__device__ __forceinline__ Ray gen_ray(const noor_camera& cam, uint32_t x, uint32_t y, Rng& rng, int rk,
+                                       RayDiff* rd = nullptr) {
     const float rx = (float)x + rng_next(rng, rk) - 0.5f;
     const float ry = (float)y + rng_next(rng, rk) - 0.5f;
     const Xf c2w = xf_from(cam.camera_to_world);
     const Xf r2c = xf_from(cam.raster_to_camera);
     f3 o = f3{0.0f, 0.0f, 0.0f}, d;
+    f3 ox, oy, dx, dy;
+    const f3 dxc = f3{cam.dx_camera.x, cam.dx_camera.y, cam.dx_camera.z};
+    const f3 dyc = f3{cam.dy_camera.x, cam.dy_camera.y, cam.dy_camera.z};
     if (cam.type == NOOR_CAMERA_ORTHO) {
         o = xf_pt(r2c, f3{rx, ry, 0.0f});
         d = f3{0.0f, 0.0f, -1.0f};
+        if (TEX) {
+            ox = add3(o, dxc);
+            oy = add3(o, dyc);
+            dx = dy = d;
+        }
     } else if (cam.type == NOOR_CAMERA_ENV) {
-        const float theta = K_PI * ry / (float)cam.h + K_PI;
-        const float phi = K_2PI * rx / (float)cam.w - K_PI_OVER_2;
-        const float sT = nm_sinf(theta), cT = nm_cosf(theta);
-        d = nrm3(nrm3(f3{sT * nm_cosf(phi), cT, sT * nm_sinf(phi)}));
+        d = nrm3(env_cam_dir(cam, rx, ry));
+        if (TEX) {
+            ox = oy = o;
+            dx = env_cam_dir(cam, rx + 1.0f, ry);
+            dy = env_cam_dir(cam, rx, ry + 1.0f);
+        }
     } else {
         const f3 pc = xf_pt(r2c, f3{rx, ry, 0.0f});
         d = nrm3(pc);
@@ -1841,7 +2162,27 @@ __device__ __forceinline__ Ray gen_ray(const noor_camera& cam, uint32_t x, uint3
             const f3 pf = add3(o, scl3(d, ft));
             o = f3{lr * pl.x, lr * pl.y, 0.0f};
             d = nrm3(sub3(pf, o));
+            if (TEX) {
+                const f3 ddx = nrm3(add3(pc, dxc));
+                const f3 pfx = add3(f3{0.0f, 0.0f, 0.0f}, scl3(ddx, -cam.focal_length / ddx.z));
+                const f3 ddy = nrm3(add3(pc, dyc));
+                const f3 pfy = add3(f3{0.0f, 0.0f, 0.0f}, scl3(ddy, -cam.focal_length / ddy.z));
+                ox = oy = o;
+                dx = nrm3(sub3(pfx, ox));
+                dy = nrm3(sub3(pfy, oy));
+            }
+        } else if (TEX) {
+            ox = oy = o;
+            dx = nrm3(add3(pc, dxc));
+            dy = nrm3(add3(pc, dyc));
         }
+    }
+    if (TEX) {
+        rd->ox = xf_pt(c2w, ox);
+        rd->oy = xf_pt(c2w, oy);
+        rd->dx = xf_vec(c2w, dx);
+        rd->dy = xf_vec(c2w, dy);
+        rd->has = 1;
     }
     return mkray(xf_pt(c2w, o), nrm3(xf_vec(c2w, d)), K_INF);
 }
@@ -2007,8 +2348,11 @@ struct Tape {
     Isect I;
 };
 
+// TEX: image textures, bump and ray differentials; `rd` holds the
+// differentials of `ray` and receives those of the next ray.
+template <bool TEX = false>
 __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, const Ray& ray, bool hit, const Hit& h, Nee& nee,
-                             f3* lookat, Tape* tp = nullptr) {
+                             f3* lookat, Tape* tp = nullptr, RayDiff* rd = nullptr) {
     const int rk = S.rng_kind;
     nee.has = 0;
     nee.na = 0;
@@ -2030,16 +2374,23 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
         return false;
     }
     Isect I;
-    make_isect(S, ray, h, I);
+    HitDiff hd;
+    make_isect<TEX>(S, ray, h, I, rd, &hd);
     if (ps.bounce == 0 && lookat) *lookat = I.p;
+    // the material with its image textures looked up (TEX), else the constants
+    DevMat mt;
+    if (TEX && !(I.type & NOOR_MAT_MESHLIGHT)) resolve_material(S, S.mats + I.mat, I.u, I.v, hd.g, &mt);
     if (is_emitter(I) && (ps.bounce == 0 || ps.specular)) {
-        const f3 Le = (I.type & NOOR_MAT_MESHLIGHT) ? light_Le(S, ray.d, (int)I.ins) : ld3(S.mats[I.mat].emittance);
+        const f3 Le = (I.type & NOOR_MAT_MESHLIGHT) ? light_Le(S, ray.d, (int)I.ins)
+                                                    : ld3(TEX ? mt.emittance : S.mats[I.mat].emittance);
         ps.L = add3(ps.L, mul3(ps.beta, Le));
         return false;
     }
+    // accumulate: bump before the BSDF is built (noor/cuda/accumulate.cuh:84-99)
+    if (TEX && rd->has && (I.type & NOOR_MAT_BUMP)) bump_frame(S, S.mats[I.mat], hd, I);
     Bsdf b;
     make_bsdf(I.type, b);
-    const DevMat& m = S.mats[I.mat];
+    const DevMat& m = (TEX && !(I.type & NOOR_MAT_MESHLIGHT)) ? mt : S.mats[I.mat];
     // sampleLight (noor/cuda/direct.cuh:33-67)
     nee.has = 1;
     nee.beta = ps.beta;
@@ -2150,6 +2501,10 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
     // spawnRay (noor/cuda/intersection.cuh:115-189)
     ps.o = dot3(I.wi, I.gn) >= 0.0f ? add3(I.p, scl3(I.gn, S.bias)) : sub3(I.p, scl3(I.gn, S.bias));
     ps.d = I.wi;
+    if (TEX) {
+        if (rd->has && (I.type & M_GLOSSY)) spawn_diff(S, I, hd, ps.o, *rd);
+        else rd->has = 0;
+    }
     return true;
 }
 

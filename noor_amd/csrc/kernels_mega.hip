@@ -41,7 +41,8 @@ __device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext
 // film (a separate instantiation keeps the film path's register budget).
 // Two waves per SIMD are required: at one the latency of the traversal's
 // node loads is exposed and the kernel runs about 1.7x slower.
-template <bool STATS, bool DIRECT>
+// TEX: image textures, alpha tests, bump and ray differentials.
+template <bool STATS, bool DIRECT, bool TEX = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_megakernel(const DevScene* __restrict__ Sp, float4* __restrict__ film, uint32_t frame0,
                                                    uint32_t spp, DevStats* __restrict__ stats,
                                                    float4* __restrict__ lookat_out, float4* __restrict__ direct) {
@@ -70,21 +71,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             ps.bounce = 0;
             ps.specular = 0;
             rng_init(ps.rng, S.rng_kind, gid, frame);
-            Ray ray = gen_ray(S.cam, x, y, ps.rng, S.rng_kind);
+            RayDiff rd;
+            Ray ray = gen_ray<TEX>(S.cam, x, y, ps.rng, S.rng_kind, &rd);
             for (int b = 0; b < S.bounces; ++b) {
                 Hit h;
                 ++ext;
-                const bool hit = bvh_closest<STATS>(S, ray, h, st, cnt, overflow);
+                const bool hit = bvh_closest<STATS, TEX>(S, ray, h, st, cnt, overflow);
                 Nee nee;
-                const bool cont = shade_bounce(S, ps, ray, hit, h, nee, center ? &lookat : nullptr);
+                const bool cont = shade_bounce<TEX>(S, ps, ray, hit, h, nee, center ? &lookat : nullptr, nullptr, &rd);
                 bool oa = false, ob = false;
                 if (nee.na) {
                     ++shd;
-                    oa = bvh_any<STATS>(S, nee.ra, st, cns, overflow);
+                    oa = bvh_any<STATS, TEX>(S, nee.ra, st, cns, overflow);
                 }
                 if (nee.nb) {
                     ++shd;
-                    ob = bvh_any<STATS>(S, nee.rb, st, cns, overflow);
+                    ob = bvh_any<STATS, TEX>(S, nee.rb, st, cns, overflow);
                 }
                 resolve_nee(ps, nee, oa, ob);
                 if (!cont) break;
@@ -114,9 +116,16 @@ hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, fl
     const size_t lds = (size_t)64 * c.S.stack_lds * sizeof(uint32_t);
     KernelTimer& T = const_cast<KernelTimer&>(c.timer);
     T.begin(c.stream, "megakernel");
+    if (direct && c.S.textured) return hipErrorNotSupported;
     if (direct)
         hipLaunchKernelGGL((k_megakernel<false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
                            spp, c.d_stats, c.d_lookat, direct);
+    else if (c.S.textured && c.opt.stats)
+        hipLaunchKernelGGL((k_megakernel<true, false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film,
+                           frame0, spp, c.d_stats, c.d_lookat, direct);
+    else if (c.S.textured)
+        hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film,
+                           frame0, spp, c.d_stats, c.d_lookat, direct);
     else if (c.opt.stats)
         hipLaunchKernelGGL((k_megakernel<true, false>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
                            spp, c.d_stats, c.d_lookat, direct);
@@ -128,7 +137,7 @@ hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, fl
 }
 
 // noor_trace: closest or any hit for a batch of rays.
-template <bool STATS>
+template <bool STATS, bool ALPHA = false>
 __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, const noor_ray* __restrict__ rays, uint32_t n,
                                               int any_hit, noor_hit* __restrict__ hits, DevStats* stats) {
     const DevScene& S = *Sp;
@@ -147,11 +156,11 @@ __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, c
         out.pad = 0;
         if (any_hit) {
             ++shd;
-            out.kind = bvh_any<STATS>(S, ray, st, cns, overflow) ? 1u : 0u;
+            out.kind = bvh_any<STATS, ALPHA>(S, ray, st, cns, overflow) ? 1u : 0u;
         } else {
             ++ext;
             Hit h;
-            if (bvh_closest<STATS>(S, ray, h, st, cnt, overflow)) {
+            if (bvh_closest<STATS, ALPHA>(S, ray, h, st, cnt, overflow)) {
                 out.t = h.t;
                 out.u = h.u;
                 out.v = h.v;
@@ -173,7 +182,13 @@ hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, in
     if (se != hipSuccess) return se;
     const size_t lds = (size_t)64 * c.S.stack_lds * sizeof(uint32_t);
     const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((n + 63) / 64, c.S.gstack_lanes / 64));
-    if (c.opt.stats)
+    if (c.S.textured && c.opt.stats)
+        hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit,
+                           d_hits, c.d_stats);
+    else if (c.S.textured)
+        hipLaunchKernelGGL((k_trace<false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit,
+                           d_hits, c.d_stats);
+    else if (c.opt.stats)
         hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit, d_hits,
                            c.d_stats);
     else

@@ -87,6 +87,13 @@ struct WaveBuffers {
     uint64_t* rng_inc = nullptr;
     float4* ro = nullptr;
     float4* rd = nullptr;
+    // ray differentials of textured scenes: auxiliary origins (w of dox = has)
+    // and directions
+    float4* dox = nullptr;
+    float4* doy = nullptr;
+    float4* ddx = nullptr;
+    float4* ddy = nullptr;
+    bool tex = false;
     float4* hit = nullptr;     // t, u, v, prim
     uint2* hit2 = nullptr;     // inst, kind
     uint32_t* cls = nullptr;   // material class of the hit (k_extend -> k_bin)
@@ -211,6 +218,22 @@ __global__ void k_stream_init(WaveBuffers B, uint32_t items) {
 // Retires the paths of the free queue and starts new items in their slots.
 // Items are handed out in free-queue order, so no atomics are needed: thread
 // t starts item item_next + t and appends its ray after the continuing rays.
+__device__ __forceinline__ void store_diff(const WaveBuffers& B, uint32_t p, const RayDiff& r) {
+    B.dox[p] = make_float4(r.ox.x, r.ox.y, r.ox.z, __int_as_float(r.has));
+    B.doy[p] = make_float4(r.oy.x, r.oy.y, r.oy.z, 0.0f);
+    B.ddx[p] = make_float4(r.dx.x, r.dx.y, r.dx.z, 0.0f);
+    B.ddy[p] = make_float4(r.dy.x, r.dy.y, r.dy.z, 0.0f);
+}
+__device__ __forceinline__ void load_diff(const WaveBuffers& B, uint32_t p, RayDiff& r) {
+    const float4 a = B.dox[p], b = B.doy[p], c = B.ddx[p], d = B.ddy[p];
+    r.ox = f3{a.x, a.y, a.z};
+    r.has = __float_as_int(a.w);
+    r.oy = f3{b.x, b.y, b.z};
+    r.dx = f3{c.x, c.y, c.z};
+    r.dy = f3{d.x, d.y, d.z};
+}
+
+template <bool TEX>
 __global__ void k_regen(const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
     const DevScene& S = *Sp;
     WaveCtl* ctl = B.ctl;
@@ -265,7 +288,9 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WaveBuffers B, int cur)
     ps.bounce = 0;
     ps.specular = 0;
     rng_init(ps.rng, S.rng_kind, gid, V.frame0 + sg.sample + V.seed);
-    const Ray r = gen_ray(V.cam, x, y, ps.rng, S.rng_kind);
+    RayDiff rdf;
+    const Ray r = gen_ray<TEX>(V.cam, x, y, ps.rng, S.rng_kind, &rdf);
+    if (TEX) store_diff(B, p, rdf);
     B.item[p] = item;
     B.pix[p] = gid;
     B.view[p] = sg.view;
@@ -296,7 +321,7 @@ __device__ __forceinline__ uint32_t refill(uint32_t* work, uint32_t count, bool 
     return idx < count ? idx : 0xFFFFFFFFu;
 }
 
-template <bool STATS>
+template <bool STATS, bool ALPHA>
 __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur, DevStats* stats) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
@@ -319,7 +344,7 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
             ++ext;
         }
         if (__ballot(have) == 0ull) break;
-        if (have && closest_steps<STATS>(S, t, st, cnt, overflow, B.steps)) {
+        if (have && closest_steps<STATS, ALPHA>(S, t, st, cnt, overflow, B.steps)) {
             B.hit[p] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.prim));
             B.hit2[p] = make_uint2(t.h.inst, t.hit ? t.h.kind : 0u);
             B.cls[p] = (uint32_t)material_class(S, t.hit, t.h);
@@ -378,6 +403,7 @@ __global__ void k_bin(WaveBuffers B, int cur) {
     B.q_class[(size_t)cls * B.capacity + slot] = p;
 }
 
+template <bool TEX>
 __global__ __launch_bounds__(256) void k_shade(const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
     const DevScene& S = *Sp;
     WaveCtl* ctl = B.ctl;
@@ -417,7 +443,9 @@ __global__ __launch_bounds__(256) void k_shade(const DevScene* __restrict__ Sp, 
         const bool hit = h2.y != 0;
         if (hit) ray.tmax = h.t;
         Nee nee;
-        cont = shade_bounce(S, ps, ray, hit, h, nee, nullptr);
+        RayDiff rdf;
+        if (TEX) load_diff(B, p, rdf);
+        cont = shade_bounce<TEX>(S, ps, ray, hit, h, nee, nullptr, nullptr, &rdf);
         // the megakernel traces at most `bounces` extension rays per path
         if (cont && ps.bounce + 1 >= (uint32_t)S.bounces) cont = false;
         ended = !cont;
@@ -434,6 +462,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevScene* __restrict__ Sp, 
             ps.bounce++;
             B.ro[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, 0.0f);
             B.rd[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
+            if (TEX) store_diff(B, p, rdf);
         }
         store_state(B, p, ps);
     }
@@ -444,7 +473,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevScene* __restrict__ Sp, 
     if (ended) B.q_free[nxt][s3] = p;
 }
 
-template <bool STATS>
+template <bool STATS, bool ALPHA>
 __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStats* stats) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
@@ -466,7 +495,7 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
             ++shd;
         }
         if (__ballot(have) == 0ull) break;
-        if (have && any_steps<STATS>(S, t, st, cnt, overflow, B.steps)) {
+        if (have && any_steps<STATS, ALPHA>(S, t, st, cnt, overflow, B.steps)) {
             Nee nee;
             const float4 a = B.nee_a[p], bt = B.nee_beta[p];
             nee.a = f3{a.x, a.y, a.z};
@@ -520,7 +549,7 @@ __global__ void k_finish(WaveBuffers B, float4* films, uint32_t s0, uint32_t s1,
 // host side
 
 static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, uint32_t max_views,
-                             uint32_t max_segments) {
+                             uint32_t max_segments, bool tex) {
     auto alloc = [&](void** p, size_t bytes) -> hipError_t {
         hipError_t e = hipMalloc(p, bytes);
         if (e == hipSuccess) B.allocs.push_back(*p);
@@ -538,6 +567,13 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
     WA(B.rng_inc, cap);
     WA(B.ro, cap);
     WA(B.rd, cap);
+    if (tex) {
+        WA(B.dox, cap);
+        WA(B.doy, cap);
+        WA(B.ddx, cap);
+        WA(B.ddy, cap);
+    }
+    B.tex = tex;
     WA(B.hit, cap);
     WA(B.hit2, cap);
     WA(B.cls, cap);
@@ -582,28 +618,29 @@ static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
     const size_t lds = (size_t)blk * B.stack_lds * sizeof(uint32_t);
     const dim3 gp(B.persist_waves / (blk / 64)), bp(blk);
     KernelTimer& T = c.timer;
+    const bool tex = c.S.textured != 0;
+    auto regen = tex ? k_regen<true> : k_regen<false>;
+    auto shade = tex ? k_shade<true> : k_shade<false>;
+    auto extend = c.opt.stats ? (tex ? k_extend<true, true> : k_extend<true, false>)
+                              : (tex ? k_extend<false, true> : k_extend<false, false>);
+    auto shadow = c.opt.stats ? (tex ? k_shadow<true, true> : k_shadow<true, false>)
+                              : (tex ? k_shadow<false, true> : k_shadow<false, false>);
     for (int i = 0; i < iters; ++i) {
         const int cur = i & 1;
         T.begin(c.stream, "wave_regen");
-        hipLaunchKernelGGL(k_regen, g1, b1, 0, c.stream, c.d_scene, B, cur);
+        hipLaunchKernelGGL(regen, g1, b1, 0, c.stream, c.d_scene, B, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_extend");
-        if (c.opt.stats)
-            hipLaunchKernelGGL(k_extend<true>, gp, bp, lds, c.stream, c.d_scene, B, cur, c.d_stats);
-        else
-            hipLaunchKernelGGL(k_extend<false>, gp, bp, lds, c.stream, c.d_scene, B, cur, c.d_stats);
+        hipLaunchKernelGGL(extend, gp, bp, lds, c.stream, c.d_scene, B, cur, c.d_stats);
         T.end(c.stream);
         T.begin(c.stream, "wave_bin");
         hipLaunchKernelGGL(k_bin, g1, b1, 0, c.stream, B, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_shade");
-        hipLaunchKernelGGL(k_shade, g1, b1, 0, c.stream, c.d_scene, B, cur);
+        hipLaunchKernelGGL(shade, g1, b1, 0, c.stream, c.d_scene, B, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_shadow");
-        if (c.opt.stats)
-            hipLaunchKernelGGL(k_shadow<true>, gp, bp, lds, c.stream, c.d_scene, B, c.d_stats);
-        else
-            hipLaunchKernelGGL(k_shadow<false>, gp, bp, lds, c.stream, c.d_scene, B, c.d_stats);
+        hipLaunchKernelGGL(shadow, gp, bp, lds, c.stream, c.d_scene, B, c.d_stats);
         T.end(c.stream);
     }
     return hipGetLastError();
@@ -650,12 +687,14 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
     capacity = std::max<uint32_t>(256u, (capacity + 255) & ~255u);
     const uint32_t seg_need = (uint32_t)std::max<uint64_t>((uint64_t)max_spp * nv, 1);
     if (!c.wave || c.wave->capacity != capacity || c.wave->sb_items < sb_need || c.wave->max_views < nv ||
-        c.wave->max_segments < seg_need || c.wave->block != block || c.wave->stack_lds != c.S.stack_lds) {
+        c.wave->max_segments < seg_need || c.wave->block != block || c.wave->stack_lds != c.S.stack_lds ||
+        c.wave->tex != (c.S.textured != 0)) {
         wave_free(c);
         c.wave = new WaveBuffers();
         c.wave->block = block;
         c.wave->stack_lds = c.S.stack_lds;
-        e = wave_alloc(*c.wave, capacity, (uint32_t)sb_need, std::max<uint32_t>(nv, 64), std::max<uint32_t>(seg_need, 256));
+        e = wave_alloc(*c.wave, capacity, (uint32_t)sb_need, std::max<uint32_t>(nv, 64), std::max<uint32_t>(seg_need, 256),
+                       c.S.textured != 0);
         if (e != hipSuccess) {
             wave_free(c);
             return e;

@@ -91,12 +91,47 @@ class Payload:
 
 def material(mtype, diffuse=(0.5, 0.5, 0.5), specular=(1.0, 1.0, 1.0), transmission=(1.0, 1.0, 1.0),
              emittance=(0.0, 0.0, 0.0), eta=(1.0, 1.0, 1.0), k=(0.0, 0.0, 0.0), roughness=(0.5, 0.5),
-             coat_ior=1.5, emitter_scale=1.0, coat_weight=0.5, coat_thickness=0.0, coat_sigma=0.0):
-    """Material description; textures are created by SceneBuilder.add_material."""
-    return dict(type=int(mtype), diffuse=diffuse, specular=specular, transmission=transmission,
+             coat_ior=1.5, emitter_scale=1.0, coat_weight=0.5, coat_thickness=0.0, coat_sigma=0.0,
+             bump=None, bumpfactor=1.0, alpha=None, uvscale=None):
+    """Material description; textures are created by SceneBuilder.add_material.
+
+    Every colour (and `roughness`) is a constant or an image: an (H, W), (H, W, 3)
+    or (H, W, 4) array with power-of-two sides, row 0 at v = 0.  `bump` is a
+    displacement image (its first channel, scaled by `bumpfactor`) and sets the
+    BUMP type bit; `alpha` is a transparency image (first channel; texels
+    <= 1e-4 cut the surface away) and sets the ALPHA bit
+    (noor/source/model.cpp:286-311).  `uvscale` maps "diffuse", "specular",
+    "emittance", "roughness", "bump" or "transparency" to the (su, sv) scale
+    of that texture's coordinates; transmission, eta and k always read at
+    scale 1 (noor/cuda/materialimp.cuh:164-186)."""
+    t = int(mtype)
+    if bump is not None:
+        t |= abi.MAT_BUMP
+    if alpha is not None:
+        t |= abi.MAT_ALPHA
+    return dict(type=t, diffuse=diffuse, specular=specular, transmission=transmission,
                 emittance=emittance, eta=eta, k=k, roughness=roughness, coat_ior=coat_ior,
                 emitter_scale=emitter_scale, coat_weight=coat_weight, coat_thickness=coat_thickness,
-                coat_sigma=coat_sigma)
+                coat_sigma=coat_sigma, bump=bump, bumpfactor=bumpfactor, alpha=alpha, uvscale=uvscale or {})
+
+
+def _is_image(v):
+    return isinstance(v, np.ndarray) and v.ndim >= 2 and v.shape[0] * v.shape[1] > 1
+
+
+def image_rgba(img):
+    """(H, W), (H, W, 1..4) -> contiguous (H, W, 4) float32; grey images fill
+    rgb, missing alpha is 1."""
+    a = np.asarray(img, np.float32)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    h, w, c = a.shape
+    out = np.ones((h, w, 4), np.float32)
+    if c == 1:
+        out[:, :, :3] = a
+    else:
+        out[:, :, :min(c, 4)] = a[:, :, :4]
+    return np.ascontiguousarray(out)
 
 
 class SceneBuilder:
@@ -127,24 +162,41 @@ class SceneBuilder:
         self.textures.append(t)
         return len(self.textures) - 1
 
+    def add_texture(self, img):
+        """Adds an image texture (see material()); returns its index."""
+        self.textures.append(image_rgba(img))
+        return len(self.textures) - 1
+
     def add_material(self, m):
         mt = abi.Material()
         for f in ("diffuse_uvscale", "specular_uvscale", "transparency_uvscale", "emittance_uvscale",
                   "bump_uvscale", "roughness_uvscale", "metalness_uvscale"):
             getattr(mt, f).x = 1.0
             getattr(mt, f).y = 1.0
-        mt.transmission_tex_idx = self._const_tex(m["transmission"])
-        mt.ior_tex_idx = self._const_tex(m["eta"])
-        mt.k_tex_idx = self._const_tex(m["k"])
-        mt.diffuse_tex_idx = self._const_tex(m["diffuse"])
+        for name, (sx, sy) in m.get("uvscale", {}).items():
+            f = getattr(mt, name + "_uvscale")
+            f.x, f.y = float(sx), float(sy)
+        tex = lambda v: self.add_texture(v) if _is_image(v) else self._const_tex(v)
+        mt.transmission_tex_idx = tex(m["transmission"])
+        mt.ior_tex_idx = tex(m["eta"])
+        mt.k_tex_idx = tex(m["k"])
+        mt.diffuse_tex_idx = tex(m["diffuse"])
         r = m["roughness"]
-        mt.roughness_tex_idx = self._const_tex((r[0], r[1], 1.0))
+        if _is_image(r):
+            ra = np.asarray(r, np.float32)
+            if ra.ndim == 2:
+                ra = np.stack([ra, ra], -1)
+            mt.roughness_tex_idx = self.add_texture(np.concatenate([ra[:, :, :2], np.ones_like(ra[:, :, :1])], -1))
+        else:
+            mt.roughness_tex_idx = self._const_tex((r[0], r[1], 1.0))
         mt.metalness_tex_idx = self._const_tex((1.0, 1.0, 1.0))
-        mt.specular_tex_idx = self._const_tex(m["specular"])
-        mt.emittance_tex_idx = self._const_tex(m["emittance"])
-        mt.transparency_tex_idx = self._const_tex((1.0, 1.0, 1.0))
-        mt.bump_tex_idx = 0
-        mt.bumpfactor = 1.0
+        mt.specular_tex_idx = tex(m["specular"])
+        mt.emittance_tex_idx = tex(m["emittance"])
+        alpha = m.get("alpha")
+        mt.transparency_tex_idx = self.add_texture(alpha) if alpha is not None else self._const_tex((1.0, 1.0, 1.0))
+        bump = m.get("bump")
+        mt.bump_tex_idx = self.add_texture(bump) if bump is not None else 0
+        mt.bumpfactor = float(m.get("bumpfactor", 1.0))
         mt.emitter_scale = m["emitter_scale"]
         mt.coat_roughness = 0.001
         mt.coat_ior = m["coat_ior"]

@@ -490,3 +490,93 @@ def showcase(seed=5, subdiv=4, width=96, height=80, spp=4, bounces=6, rr=3, mis=
     spec = make_spec(payload, bounces=bounces, rr=rr, skydome_type=abi.SKY_HDR, mis=mis)
     opt = make_options(rng=rng_kind, compat=compat)
     return Workload("showcase", payload, cam, spec, opt, None, spp, info=dict(triangles=payload.view.num_triangles))
+
+
+# ---------------------------------------------------------------------------------------------
+# image textures
+
+def checker(n, checks, c0=(0.8, 0.8, 0.8), c1=(0.1, 0.1, 0.1)):
+    """n x n RGB checkerboard of checks x checks squares."""
+    i = (np.arange(n) * checks // n)
+    m = ((i[:, None] + i[None, :]) & 1).astype(np.float32)[:, :, None]
+    return np.asarray(c0, np.float32) * (1 - m) + np.asarray(c1, np.float32) * m
+
+
+def noise_image(n, seed, freq=8.0):
+    """n x n value-noise image in [0, 1], periodic in both directions."""
+    t = np.arange(n) / n
+    X, Z = np.meshgrid(t, t, indexing="ij")
+    # sum of periodic value-noise octaves: sample on a torus so the image tiles
+    a = 2 * np.pi * X
+    b = 2 * np.pi * Z
+    p = np.stack([np.cos(a), np.sin(a), np.cos(b) + 2 * np.sin(b)], -1).reshape(-1, 3)
+    v = value_noise3(p * freq / (2 * np.pi), seed).reshape(n, n)
+    v = (v - v.min()) / max(v.max() - v.min(), 1e-12)
+    return v.astype(np.float32)
+
+
+def quad_uv(corner, u, v):
+    """quad() with texture coordinates (0,0), (1,0), (1,1), (0,1) at corner,
+    corner + u, corner + u + v, corner + v."""
+    pos, tri = quad(corner, u, v)
+    return pos, tri, np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float64)
+
+
+def textured(seed=8, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, lens_radius=0.0,
+             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL, tex=64):
+    """The image-texture pipeline of the reference: mip-mapped colour,
+    roughness and emittance textures read at the pixel footprint (ray
+    differentials from the camera, carried through mirror, metal and glass
+    bounces), bump maps and alpha-cut surfaces, in the open Cornell geometry
+    under an HDR environment."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    ck = checker(tex, 8)
+    bumpimg = noise_image(tex, seed + 1)
+    stripes = np.zeros((tex // 2, tex // 2, 3), np.float32)
+    stripes[:, :, 0] = 0.2 + 0.6 * (np.arange(tex // 2)[None, :] % 8 < 4)
+    stripes[:, :, 1] = 0.3
+    stripes[:, :, 2] = 0.2 + 0.5 * noise_image(tex // 2, seed + 2)
+    rough = 0.05 + 0.5 * noise_image(tex // 2, seed + 3)
+    yy, xx = np.mgrid[0:tex, 0:tex] / tex
+    holes = (((xx * 4) % 1 - 0.5) ** 2 + ((yy * 4) % 1 - 0.5) ** 2 > 0.09).astype(np.float32)
+    glow = np.zeros((16, 16, 3), np.float32)
+    glow[:, :, :] = (np.arange(16)[:, None, None] % 4 < 2) * np.array([6.0, 5.0, 3.0], np.float32)
+    mats = dict(
+        floor=sb.add_material(material(abi.MAT_DIFFUSE, diffuse=ck, bump=bumpimg, bumpfactor=0.02,
+                                       uvscale={"diffuse": (4, 4), "bump": (2, 2)})),
+        back=sb.add_material(material(abi.MAT_GLOSSY, diffuse=stripes, specular=(0.3, 0.3, 0.3), roughness=rough,
+                                      uvscale={"diffuse": (2, 1), "roughness": (1, 2)})),
+        left=sb.add_material(material(abi.MAT_METAL, eta=CU_ETA, k=CU_K, roughness=rough * 0.5)),
+        right=sb.add_material(material(abi.MAT_MIRROR, specular=(0.9, 0.9, 0.9))),
+        cut=sb.add_material(material(abi.MAT_DIFFUSE, diffuse=ck[:, :, ::-1].copy(), alpha=holes,
+                                     uvscale={"transparency": (1, 1)})),
+        glass=sb.add_material(material(abi.MAT_GLASS, specular=(1, 1, 1), transmission=(1, 1, 1), coat_ior=1.5)),
+        torus=sb.add_material(material(abi.MAT_SUBSTRATE, diffuse=stripes, specular=(0.05, 0.05, 0.05),
+                                       roughness=(0.15, 0.15), bump=bumpimg, bumpfactor=0.01,
+                                       uvscale={"diffuse": (4, 1), "bump": (8, 2)})),
+        glow=sb.add_material(material(abi.MAT_EMITTER, emittance=glow, emitter_scale=1.5)),
+    )
+    walls = [
+        (quad_uv((-1.0, -0.5, -1.5), (2.0, 0.0, 0.0), (0.0, 0.0, 2.0)), mats["floor"]),    # floor, normal +y
+        (quad_uv((-1.0, -0.5, -1.0), (0.0, 1.5, 0.0), (2.0, 0.0, 0.0)), mats["back"]),     # back, normal +z
+        (quad_uv((-1.0, -0.5, -1.5), (0.0, 0.0, 2.0), (0.0, 1.5, 0.0)), mats["left"]),     # left, normal +x
+        (quad_uv((1.0, -0.5, -1.5), (0.0, 1.5, 0.0), (0.0, 0.0, 2.0)), mats["right"]),     # right, normal -x
+        (quad_uv((-0.55, -0.45, -0.6), (0.0, 0.6, 0.0), (0.5, 0.0, 0.0)), mats["cut"]),    # alpha-cut panel
+        (quad_uv((0.2, 0.55, -0.95), (0.5, 0.0, 0.0), (0.0, 0.25, 0.0)), mats["glow"]),   # textured emitter
+    ]
+    for (pos, tri, uv), m in walls:
+        sb.add_mesh(Mesh(pos, tri, uvs=uv, material=m))
+    sv, sf = icosphere(3)
+    sb.add_mesh(Mesh(sv * 0.2 + np.array([0.35, -0.28, -0.45]), sf, material=mats["glass"]))
+    tp, tt, tn, tu = torus(0.22, 0.08, 48, 24)
+    R = random_rotation(rng)
+    sb.add_mesh(Mesh(tp @ R.T + np.array([-0.1, -0.3, -0.75]), tt, normals=tn @ R.T, uvs=tu, material=mats["torus"]))
+    sb.set_env(env_map(128, 64, seed + 3, lobes=2, sigma_deg=8.0, peak=20.0))
+    payload = sb.build()
+    cam = make_camera((0.0, 0.0, 0.95), (0.0, -0.15, -1.0), (0, 1, 0), np.radians(60.0), width, height,
+                      lens_radius=lens_radius)
+    spec = make_spec(payload, bounces=bounces, rr=rr, skydome_type=abi.SKY_HDR, mis=mis)
+    opt = make_options(rng=rng_kind, compat=compat)
+    return Workload("textured", payload, cam, spec, opt, None, spp, info=dict(triangles=payload.view.num_triangles))
+

@@ -55,6 +55,12 @@ def lib():
     L.oracle_trace_brute_f32.argtypes = [vp, P(abi.Ray), C.c_uint32, P(abi.Hit)]
     L.oracle_camera_ray_f32.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(abi.Ray)]
     L.oracle_render_detached_f64.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int, P(C.c_double)]
+    L.oracle_texture_desc.argtypes = [vp, C.c_uint32, P(C.c_uint32 * 4)]
+    L.oracle_texture_texels.restype = P(C.c_float)
+    L.oracle_texture_texels.argtypes = [vp]
+    L.oracle_tex_lookup_f32.argtypes = [vp, C.c_uint32, C.c_int, C.c_float, C.c_float, P(C.c_float), P(C.c_float)]
+    L.oracle_camera_ray_diff_f32.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float)]
+    L.oracle_hit_diff_f32.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float)]
     L.oracle_tea16.restype = C.c_uint32
     L.oracle_tea16.argtypes = [C.c_uint32, C.c_uint32]
     L.oracle_tea.restype = C.c_uint32
@@ -155,6 +161,44 @@ class OracleScene:
         r = abi.Ray()
         lib().oracle_camera_ray_f32(self._h, x, y, frame, C.byref(r))
         return r
+
+    def camera_ray_diff(self, x, y, frame):
+        """(o, d, ox, oy, dx, dy) of the camera ray with its differentials."""
+        out = np.zeros(18, np.float32)
+        lib().oracle_camera_ray_diff_f32(self._h, x, y, frame, _fp(out))
+        return out.reshape(6, 3)
+
+    def hit_diff(self, x, y, frame):
+        """First hit of the camera ray with its differentials as a dict, or None."""
+        out = np.zeros(18, np.float32)
+        if not lib().oracle_hit_diff_f32(self._h, x, y, frame, _fp(out)):
+            return None
+        return dict(uv=out[0:2], g=out[2:6], p=out[6:9], dpdx=out[9:12], dpdy=out[12:15], gn=out[15:18])
+
+    def texture_levels(self, i):
+        """The mip chain of store texture i: a list of (h, w, 4) arrays."""
+        d = (C.c_uint32 * 4)()
+        if lib().oracle_texture_desc(self._h, i, C.byref(d)) != 0:
+            raise IndexError(i)
+        off, w, h, levels = list(d)
+        n, ww, hh = 0, w, h
+        for _ in range(levels):
+            n += ww * hh
+            ww, hh = max(1, ww // 2), max(1, hh // 2)
+        flat = np.ctypeslib.as_array(lib().oracle_texture_texels(self._h), shape=(4 * (off + n),))
+        out, o, ww, hh = [], 4 * off, w, h
+        for _ in range(levels):
+            out.append(flat[o:o + 4 * ww * hh].reshape(hh, ww, 4).copy())
+            o += 4 * ww * hh
+            ww, hh = max(1, ww // 2), max(1, hh // 2)
+        return out
+
+    def tex_lookup(self, i, kind, u, v, g=(0.0, 0.0, 0.0, 0.0)):
+        """kind 0: tex2D (level 0), 1: tex2DGrad with g = (dudx, dvdx, dudy, dvdy), 2: point."""
+        gg = np.asarray(g, np.float32)
+        out = np.zeros(4, np.float32)
+        lib().oracle_tex_lookup_f32(self._h, i, kind, float(u), float(v), _fp(gg), _fp(out))
+        return out
 
     def dist(self):
         w = C.c_int()

@@ -30,7 +30,13 @@
 extern "C" {
 #endif
 
-/* Resolved material: every 1x1 texture read once (noor/cuda/materialimp.cuh:134-214). */
+/* Material fields that can read an image texture (noor/cuda/materialimp.cuh:120-186). */
+enum { ORC_TX_DIFFUSE = 0, ORC_TX_SPECULAR, ORC_TX_TRANSMISSION, ORC_TX_EMITTANCE, ORC_TX_ETA, ORC_TX_K,
+       ORC_TX_ROUGH, ORC_TX_BUMP, ORC_TX_ALPHA, ORC_TX_COUNT };
+
+/* Material: every 1x1 texture read once (noor/cuda/materialimp.cuh:134-214);
+ * field f with bit f of tex_mask reads texture tex[f] of the store at uv
+ * scale uvs[f] instead (image textures, and every bump and transparency map). */
 typedef struct orc_material {
     uint32_t type;
     float diffuse[3], specular[3], transmission[3], emittance[3];
@@ -38,7 +44,17 @@ typedef struct orc_material {
     float rough[2];
     float coat_ior;
     float coat_weight, coat_thickness, coat_sigma;
+    uint32_t tex_mask;
+    uint32_t tex[ORC_TX_COUNT];
+    float uvs[ORC_TX_COUNT][2];
+    float bumpfactor, emitter_scale;
 } orc_material;
+
+/* One texture of the store: level l is max(1, w >> l) x max(1, h >> l) RGBA
+ * texels, the levels packed from texel `offset`. */
+typedef struct orc_texdesc {
+    uint32_t offset, w, h, levels;
+} orc_texdesc;
 
 /* 2D piecewise-constant distribution in the reference's single-buffer layout
  * (noor/cuda/distribution2d.cuh:165-200): rows of W+1 floats, the marginal
@@ -62,6 +78,11 @@ typedef struct orc_scene {
     float* sky_bake;             /* PHYSICAL: sin(theta) * sky baked texture */
     uint32_t sky_w, sky_h;
     orc_dist2d dist;
+    /* material texture store: mip chains (CudaMipMap, noor/cuda/texture.cuh:226-400) */
+    float* tex_texels;           /* RGBA */
+    orc_texdesc* tex_desc;
+    uint32_t num_tex;
+    int textured;                /* any material reads a texture of the store */
     int num_lights;              /* all lights incl. the infinite one */
     int sky_enabled;
     int stack_size;
@@ -116,6 +137,22 @@ int oracle_trace_f64(orc_scene* s, const noor_ray* rays, uint32_t n, int any_hit
 int oracle_trace_brute_f32(orc_scene* s, const noor_ray* rays, uint32_t n, noor_hit* hits);
 /* Generate the camera ray of pixel (x, y) for frame (test hook). */
 void oracle_camera_ray_f32(orc_scene* s, uint32_t x, uint32_t y, uint32_t frame, noor_ray* out);
+
+/* Texture store hooks: the descriptor of store texture i (0 on success),
+ * its texels (all levels), and one lookup as the shading code makes it:
+ * kind 0 tex2D (level 0), 1 tex2DGrad with the uv gradients g = (dudx, dvdx,
+ * dudy, dvdy), 2 point-filtered. */
+int oracle_texture_desc(const orc_scene* s, uint32_t i, orc_texdesc* out);
+const float* oracle_texture_texels(const orc_scene* s);
+void oracle_tex_lookup_f32(const orc_scene* s, uint32_t i, int kind, float u, float v, const float g[4],
+                           float out[4]);
+/* Camera ray of pixel (x, y) for `frame` with its differentials:
+ * out = o, d, ox, oy, dx, dy (18 floats) (test hook). */
+void oracle_camera_ray_diff_f32(orc_scene* s, uint32_t x, uint32_t y, uint32_t frame, float out[18]);
+/* The camera ray's first surface hit with its differentials: out = u, v,
+ * dudx, dvdx, dudy, dvdy, p, dpdx, dpdy, gn; returns 0 when it hits no
+ * triangle (test hook). */
+int oracle_hit_diff_f32(orc_scene* s, uint32_t x, uint32_t y, uint32_t frame, float out[18]);
 
 /* Detached-sampling render for finite differences (FP64): sampling decisions
  * use `s0` (frozen parameters), evaluation uses `s` (perturbed parameters). */

@@ -251,9 +251,130 @@ static void tex_rgb(const noor_payload_view* p, uint32_t idx, float out[3]) {
     out[2] = t->rgba[2];
 }
 
-static void resolve_material(const noor_payload_view* p, const noor_material* m, orc_material* o) {
+/* ---- material texture store (noor/cuda/texture.cuh:204-400) ------------------ */
+
+static uint32_t mip_levels(uint32_t w, uint32_t h) {
+    uint32_t sz = w > h ? w : h, levels = 0;
+    while (sz) { sz /= 2; ++levels; }   /* CudaMipMap::getMipMapLevels */
+    return levels;
+}
+
+static int wrapi(int x, int n) {
+    const int r = x % n;
+    return r < 0 ? r + n : r;
+}
+
+/* tex2D, linear filtering, wrap addressing, normalized (s, q) on one level */
+static void level_bilinear(const float* lvl, uint32_t w, uint32_t h, float s, float q, float out[4]) {
+    if (!(fabsf(s) < 1e30f)) s = 0.0f;
+    if (!(fabsf(q) < 1e30f)) q = 0.0f;
+    s = s - floorf(s);
+    q = q - floorf(q);
+    const float x = s * (float)w - 0.5f, y = q * (float)h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float ax = x - fx, ay = y - fy;
+    const int x0 = wrapi((int)fx, (int)w), x1 = wrapi((int)fx + 1, (int)w);
+    const int y0 = wrapi((int)fy, (int)h), y1 = wrapi((int)fy + 1, (int)h);
+    const float* t00 = lvl + 4 * ((size_t)y0 * w + x0);
+    const float* t10 = lvl + 4 * ((size_t)y0 * w + x1);
+    const float* t01 = lvl + 4 * ((size_t)y1 * w + x0);
+    const float* t11 = lvl + 4 * ((size_t)y1 * w + x1);
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    for (int c = 0; c < 4; ++c)
+        out[c] = by * (bx * t00[c] + ax * t10[c]) + ay * (bx * t01[c] + ax * t11[c]);
+}
+
+/* Appends a texture and its mip chain: each texel of the next level is the
+ * mean of the four samples of this level at ((x + i) / W, (y + j) / H),
+ * i, j in {0, 1}, (W, H) the next level's size (the mipmap kernel,
+ * noor/cuda/texture.cuh:204-221, driven by generateMipMaps, 322-388). */
+static uint32_t store_add(orc_scene* s, const noor_texture_view* t, size_t* cap) {
+    const uint32_t levels = mip_levels(t->width, t->height);
+    size_t need = 0;
+    uint32_t w = t->width, h = t->height;
+    for (uint32_t l = 0; l < levels; ++l) {
+        need += (size_t)w * h;
+        w = w > 1 ? w / 2 : 1;
+        h = h > 1 ? h / 2 : 1;
+    }
+    orc_texdesc d = {0, t->width, t->height, levels};
+    size_t used = 0;
+    for (uint32_t i = 0; i < s->num_tex; ++i) {
+        size_t n = 0;
+        uint32_t a = s->tex_desc[i].w, b = s->tex_desc[i].h;
+        for (uint32_t l = 0; l < s->tex_desc[i].levels; ++l) {
+            n += (size_t)a * b;
+            a = a > 1 ? a / 2 : 1;
+            b = b > 1 ? b / 2 : 1;
+        }
+        used += n;
+    }
+    if (used + need > *cap) {
+        *cap = 2 * (used + need);
+        s->tex_texels = (float*)realloc(s->tex_texels, sizeof(float) * 4 * *cap);
+    }
+    s->tex_desc = (orc_texdesc*)realloc(s->tex_desc, sizeof(orc_texdesc) * (s->num_tex + 1));
+    d.offset = (uint32_t)used;
+    float* base = s->tex_texels + 4 * used;
+    memcpy(base, t->rgba, sizeof(float) * 4 * (size_t)t->width * t->height);
+    float* src = base;
+    uint32_t sw = t->width, sh = t->height;
+    for (uint32_t l = 1; l < levels; ++l) {
+        const uint32_t W = sw > 1 ? sw / 2 : 1, H = sh > 1 ? sh / 2 : 1;
+        float* dst = src + 4 * (size_t)sw * sh;
+        const float px = (float)(1.0 / (double)W), py = (float)(1.0 / (double)H);
+        for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) {
+                float a[4], b[4], c[4], e[4];
+                level_bilinear(src, sw, sh, (float)(x + 0) * px, (float)(y + 0) * py, a);
+                level_bilinear(src, sw, sh, (float)(x + 1) * px, (float)(y + 0) * py, b);
+                level_bilinear(src, sw, sh, (float)(x + 1) * px, (float)(y + 1) * py, c);
+                level_bilinear(src, sw, sh, (float)(x + 0) * px, (float)(y + 1) * py, e);
+                float* o = dst + 4 * ((size_t)y * W + x);
+                for (int k = 0; k < 4; ++k) o[k] = (((a[k] + b[k]) + c[k]) + e[k]) / 4.0f;
+            }
+        src = dst;
+        sw = W;
+        sh = H;
+    }
+    s->tex_desc[s->num_tex] = d;
+    return s->num_tex++;
+}
+
+int oracle_texture_desc(const orc_scene* s, uint32_t i, orc_texdesc* out) {
+    if (i >= s->num_tex) return -1;
+    *out = s->tex_desc[i];
+    return 0;
+}
+const float* oracle_texture_texels(const orc_scene* s) { return s->tex_texels; }
+
+static void resolve_material(orc_scene* s, const noor_payload_view* p, const noor_material* m, orc_material* o,
+                             int64_t* store_idx, size_t* cap) {
     float tmp[3];
     o->type = m->type;
+    const float one[2] = {1.0f, 1.0f};
+    const uint32_t tex[ORC_TX_COUNT] = {m->diffuse_tex_idx, m->specular_tex_idx, m->transmission_tex_idx,
+                                        m->emittance_tex_idx, m->ior_tex_idx, m->k_tex_idx,
+                                        m->roughness_tex_idx, m->bump_tex_idx, m->transparency_tex_idx};
+    const float* uvs[ORC_TX_COUNT] = {&m->diffuse_uvscale.x, &m->specular_uvscale.x, one,
+                                      &m->emittance_uvscale.x, one, one, &m->roughness_uvscale.x,
+                                      &m->bump_uvscale.x, &m->transparency_uvscale.x};
+    for (int f = 0; f < ORC_TX_COUNT; ++f) {
+        if (f == ORC_TX_BUMP && !(m->type & NOOR_MAT_BUMP)) continue;
+        if (f == ORC_TX_ALPHA && !(m->type & NOOR_MAT_ALPHA)) continue;
+        if (tex[f] >= p->num_textures) continue;
+        const noor_texture_view* t = &p->textures[tex[f]];
+        if (!t->rgba || t->width == 0 || t->height == 0) continue;
+        if (f < ORC_TX_BUMP && t->width * t->height == 1) continue;
+        if (store_idx[tex[f]] < 0) store_idx[tex[f]] = store_add(s, t, cap);
+        o->tex_mask |= 1u << f;
+        o->tex[f] = (uint32_t)store_idx[tex[f]];
+        o->uvs[f][0] = uvs[f][0];
+        o->uvs[f][1] = uvs[f][1];
+    }
+    o->bumpfactor = m->bumpfactor;
+    o->emitter_scale = m->emitter_scale;
+    if (o->tex_mask) s->textured = 1;
     tex_rgb(p, m->diffuse_tex_idx, o->diffuse);
     tex_rgb(p, m->specular_tex_idx, o->specular);
     tex_rgb(p, m->transmission_tex_idx, o->transmission);
@@ -320,7 +441,14 @@ int oracle_scene_create(const noor_payload_view* p, const noor_hosek* hosek,
     s->spec = *spec;
     s->opt = *opt;
     s->mats = (orc_material*)calloc(p->num_materials ? p->num_materials : 1, sizeof(orc_material));
-    for (uint32_t i = 0; i < p->num_materials; ++i) resolve_material(p, &p->materials[i], &s->mats[i]);
+    {
+        int64_t* store_idx = (int64_t*)malloc(sizeof(int64_t) * (p->num_textures ? p->num_textures : 1));
+        for (uint32_t i = 0; i < p->num_textures; ++i) store_idx[i] = -1;
+        size_t cap = 0;
+        for (uint32_t i = 0; i < p->num_materials; ++i)
+            resolve_material(s, p, &p->materials[i], &s->mats[i], store_idx, &cap);
+        free(store_idx);
+    }
     if (p->num_textures > 0) {
         const noor_texture_view* t = &p->textures[p->num_textures - 1];
         s->env_w = t->width;
@@ -341,6 +469,8 @@ void oracle_scene_destroy(orc_scene* s) {
     oracle_dist_free(&s->dist);
     free(s->sky_bake);
     free(s->mats);
+    free(s->tex_texels);
+    free(s->tex_desc);
     free(s);
 }
 
