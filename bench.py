@@ -142,6 +142,58 @@ def bench_batch(log_prefix="batch"):
                          "best": {"queue_capacity": best[0], "block": best[1], "stack_lds": best[2]}}}
 
 
+def bench_textured(width=1920, height=1080, spp=8, tex=512):
+    """The image-texture pipeline at 1080p: the textured scene (mip-mapped
+    colour, roughness and emittance textures at the ray-differential
+    footprint, bump and alpha maps) against its untextured twin (every image
+    replaced by its mean), megakernel and wavefront, with the films of the two
+    renderers compared."""
+    import numpy as np
+    from noor_amd import scenes
+    from noor_amd.renderer import Renderer
+    res = {}
+    for name, flat in (("textured", False), ("flat", True)):
+        wl = scenes.textured(width=width, height=height, tex=tex, flat=flat)
+        r = Renderer.from_workload(wl)
+        wl.options.stats = 1
+        r.update_options(wl.options)
+        r.reset_stats()
+        r.render(spp=spp, frame=1)
+        st = r.stats()
+        rays = st["extension_rays"] + st["shadow_rays"]
+        wl.options.stats = 0
+        r.update_options(wl.options)
+        row = {"rays": rays}
+        films = {}
+        for mode in ("megakernel", "wavefront"):
+            go = r.render if mode == "megakernel" else r.render_wavefront
+            go(spp=spp, frame=1)  # warm-up
+            r.synchronize()
+            t = time.perf_counter()
+            go(spp=spp, frame=1)
+            r.synchronize()
+            dt = time.perf_counter() - t
+            films[mode] = r.film()
+            row[mode] = {"ms": round(dt * 1e3, 2), "mrays_per_s": round(rays / dt / 1e6, 1)}
+        row["wavefront_equals_megakernel"] = bool(np.array_equal(films["megakernel"], films["wavefront"]))
+        r.set_timing(True)
+        r.reset_kernel_times()
+        r.render_wavefront(spp=spp, frame=1)
+        r.synchronize()
+        kt = r.kernel_times()
+        r.set_timing(False)
+        row["wave_kernel_ms"] = {k: round(v[1], 3) for k, v in kt.items() if k.startswith("wave_")}
+        res[name] = row
+        log(f"textured leg ({name}): {rays} rays, megakernel {row['megakernel']['mrays_per_s']} Mrays/s, "
+            f"wavefront {row['wavefront']['mrays_per_s']} Mrays/s, films equal {row['wavefront_equals_megakernel']}")
+        r.close()
+    res["workload"] = (f"textured open Cornell scene, {width}x{height}, {spp} spp, 6 bounces, {tex}^2 textures; "
+                       f"flat: the same scene with every image replaced by its mean")
+    res["wavefront_cost_of_textures"] = round(res["flat"]["wavefront"]["mrays_per_s"] /
+                                              res["textured"]["wavefront"]["mrays_per_s"], 3)
+    return res
+
+
 def bench_adjoint(passes=10):
     """Forward (one-sample render) and path-replay backward on the 512^2,
     depth-4 Cornell scene with 64 glossy materials and a 512x256 environment."""
@@ -221,6 +273,7 @@ def main():
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-textured", action="store_true")
     ap.add_argument("--no-build-compare", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
     args = ap.parse_args()
@@ -393,6 +446,8 @@ def main():
         out["batch"] = bench_batch()
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint()
+    if not args.no_textured:
+        out["textured"] = bench_textured()
     print(json.dumps(out), flush=True)
 
 

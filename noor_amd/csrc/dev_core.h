@@ -1016,20 +1016,26 @@ struct UvGrad {
     float dudx, dvdx, dudy, dvdy;
 };
 
-// MaterialManager getters: the material with its image textures looked up at
-// (u, v), written to `out` (noor/cuda/materialimp.cuh:120-186,
-// noor/cuda/texture.cuh:114-138: the uv scaled component-wise, the x and y
-// gradients scaled by uvscale.x and uvscale.y).
-__device__ __noinline__ void resolve_material(const DevScene& S, const DevMat* base, float u, float v, UvGrad g,
-                                              DevMat* out) {
-    DevMat m = *base;
+// One texture lookup as the material getters make it: tex2D at level 0
+// (lod0) or tex2DGrad.  Out of line: the shading kernels call it from
+// several sites.
+__device__ __noinline__ float4 tex_field(const DevScene& S, uint32_t ti, float s, float q, float gx0, float gx1,
+                                         float gy0, float gy1, bool lod0) {
+    return lod0 ? tex_eval(S, ti, s, q) : tex_grad(S, ti, s, q, gx0, gx1, gy0, gy1);
+}
+
+// MaterialManager getters: the fields of `m` that read image textures looked
+// up at (u, v) (noor/cuda/materialimp.cuh:120-186, noor/cuda/texture.cuh:114-138:
+// the uv scaled component-wise, the x and y gradients scaled by uvscale.x
+// and uvscale.y; roughness reads level 0).
+__device__ __forceinline__ void resolve_material(const DevScene& S, float u, float v, UvGrad g, DevMat& m) {
     const uint32_t mask = m.tex_mask;
-    for (uint32_t f = 0; f < TX_COUNT; ++f) {
-        if (!(mask & (1u << f)) || f == TX_BUMP || f == TX_ALPHA) continue;
+#pragma unroll
+    for (uint32_t f = 0; f < TX_BUMP; ++f) {
+        if (!(mask & (1u << f))) continue;
         const float sx = m.uvs[f][0], sy = m.uvs[f][1];
-        const float4 c = (f == TX_ROUGH) ? tex_eval(S, m.tex[f], sx * u, sy * v)
-                                         : tex_grad(S, m.tex[f], sx * u, sy * v, sx * g.dudx, sx * g.dvdx,
-                                                    sy * g.dudy, sy * g.dvdy);
+        const float4 c = tex_field(S, m.tex[f], sx * u, sy * v, sx * g.dudx, sx * g.dvdx, sy * g.dudy, sy * g.dvdy,
+                                   f == TX_ROUGH);
         switch (f) {
             case TX_DIFFUSE: m.diffuse[0] = c.x; m.diffuse[1] = c.y; m.diffuse[2] = c.z; break;
             case TX_SPECULAR: m.specular[0] = c.x; m.specular[1] = c.y; m.specular[2] = c.z; break;
@@ -1045,7 +1051,6 @@ __device__ __noinline__ void resolve_material(const DevScene& S, const DevMat* b
             default: break;
         }
     }
-    *out = m;
 }
 
 // Hosek-Wilkie (noor/cuda/hosek.cuh:55-101)
@@ -1259,20 +1264,26 @@ __device__ __forceinline__ float bump_tex(const DevScene& S, const DevMat& m, fl
     const float sx = m.uvs[TX_BUMP][0], sy = m.uvs[TX_BUMP][1];
     return tex_grad(S, m.tex[TX_BUMP], sx * u, sy * v, sx * g.dudx, sx * g.dvdx, sy * g.dudy, sy * g.dvdy).x;
 }
-__device__ __noinline__ void bump_frame(const DevScene& S, const DevMat& m, const HitDiff& hd, Isect& I) {
-    float du = 0.5f * (fabsf(hd.g.dudx) + fabsf(hd.g.dudy));
-    float dv = 0.5f * (fabsf(hd.g.dvdx) + fabsf(hd.g.dvdy));
+struct Frame {
+    f3 ss, ts, sn;
+};
+__device__ __noinline__ Frame bump_frame(const DevScene& S, const DevMat& m, float u, float v, UvGrad g, Frame F,
+                                         f3 gn) {
+    float du = 0.5f * (fabsf(g.dudx) + fabsf(g.dudy));
+    float dv = 0.5f * (fabsf(g.dvdx) + fabsf(g.dvdy));
     du = du == 0.0f ? 0.001f : du;
     dv = dv == 0.0f ? 0.001f : dv;
-    const float disp = bump_tex(S, m, I.u, I.v, hd.g);
-    const float ud = m.bumpfactor * (bump_tex(S, m, I.u + du, I.v, hd.g) - disp) / du;
-    const float vd = m.bumpfactor * (bump_tex(S, m, I.u, I.v + dv, hd.g) - disp) / dv;
-    const f3 dpdu = add3(I.ss, scl3(I.sn, ud));
-    const f3 dpdv = add3(I.ts, scl3(I.sn, vd));
+    const float disp = bump_tex(S, m, u, v, g);
+    const float ud = m.bumpfactor * (bump_tex(S, m, u + du, v, g) - disp) / du;
+    const float vd = m.bumpfactor * (bump_tex(S, m, u, v + dv, g) - disp) / dv;
+    const f3 dpdu = add3(F.ss, scl3(F.sn, ud));
+    const f3 dpdv = add3(F.ts, scl3(F.sn, vd));
     const f3 n = nrm3s(cross3(dpdu, dpdv));
-    I.ss = nrm3s(sub3(dpdu, scl3(n, dot3(n, dpdu))));
-    I.ts = nrm3s(cross3(n, I.ss));
-    I.sn = dot3(I.gn, n) < 0.0f ? scl3(n, -1.0f) : n;
+    Frame o;
+    o.ss = nrm3s(sub3(dpdu, scl3(n, dot3(n, dpdu))));
+    o.ts = nrm3s(cross3(n, o.ss));
+    o.sn = dot3(gn, n) < 0.0f ? scl3(n, -1.0f) : n;
+    return o;
 }
 
 // spawnRay with differentials (noor/cuda/intersection.cuh:115-182): the
@@ -1617,17 +1628,73 @@ __device__ __forceinline__ float coat_pdf(const DevMat& m, f3 wo, f3 wi) {
 // VGPRs, one wave per SIMD instead of two); as calls they cost only the lanes
 // whose material has them.  Values are passed by value and the material by
 // its global-memory address, so the call spills nothing to the stack.
-__device__ __noinline__ f3 rare_eval(int id, const DevMat* m, f3 wo, f3 wi) {
-    if (id == L_MICRO_T) return micro_t_eval(*m, wo, wi);
-    if (id == L_FRESNEL_BLEND) return blend_eval(*m, wo, wi);
-    return coat_eval(*m, wo, wi);
+__device__ __forceinline__ f3 rare_eval_inl(int id, const DevMat& m, f3 wo, f3 wi) {
+    if (id == L_MICRO_T) return micro_t_eval(m, wo, wi);
+    if (id == L_FRESNEL_BLEND) return blend_eval(m, wo, wi);
+    return coat_eval(m, wo, wi);
 }
-__device__ __noinline__ float rare_pdf(int id, const DevMat* m, f3 wo, f3 wi) {
-    if (id == L_MICRO_T) return micro_t_pdf(*m, wo, wi);
-    if (id == L_FRESNEL_BLEND) return blend_pdf(*m, wo, wi);
-    return coat_pdf(*m, wo, wi);
+__device__ __forceinline__ float rare_pdf_inl(int id, const DevMat& m, f3 wo, f3 wi) {
+    if (id == L_MICRO_T) return micro_t_pdf(m, wo, wi);
+    if (id == L_FRESNEL_BLEND) return blend_pdf(m, wo, wi);
+    return coat_pdf(m, wo, wi);
+}
+__device__ __noinline__ f3 rare_eval(int id, const DevMat* m, f3 wo, f3 wi) { return rare_eval_inl(id, *m, wo, wi); }
+__device__ __noinline__ float rare_pdf(int id, const DevMat* m, f3 wo, f3 wi) { return rare_pdf_inl(id, *m, wo, wi); }
+// The same for a material held in registers (textured shading resolves its
+// material per hit): the fields the rare lobes read, passed by value, so the
+// caller's copy need not live in memory.
+struct RareMat {
+    float diffuse[3], specular[3], transmission[3], eta[3], k[3], rough[2];
+    float coat_ior, coat_weight, coat_thickness, coat_sigma;
+};
+__device__ __forceinline__ RareMat rare_pack(const DevMat& m) {
+    RareMat r;
+    for (int c = 0; c < 3; ++c) {
+        r.diffuse[c] = m.diffuse[c];
+        r.specular[c] = m.specular[c];
+        r.transmission[c] = m.transmission[c];
+        r.eta[c] = m.eta[c];
+        r.k[c] = m.k[c];
+    }
+    r.rough[0] = m.rough[0];
+    r.rough[1] = m.rough[1];
+    r.coat_ior = m.coat_ior;
+    r.coat_weight = m.coat_weight;
+    r.coat_thickness = m.coat_thickness;
+    r.coat_sigma = m.coat_sigma;
+    return r;
+}
+__device__ __forceinline__ DevMat rare_unpack(const RareMat& r) {
+    DevMat m;
+    for (int c = 0; c < 3; ++c) {
+        m.diffuse[c] = r.diffuse[c];
+        m.specular[c] = r.specular[c];
+        m.transmission[c] = r.transmission[c];
+        m.eta[c] = r.eta[c];
+        m.k[c] = r.k[c];
+        m.emittance[c] = 0.0f;
+    }
+    m.rough[0] = r.rough[0];
+    m.rough[1] = r.rough[1];
+    m.coat_ior = r.coat_ior;
+    m.coat_weight = r.coat_weight;
+    m.coat_thickness = r.coat_thickness;
+    m.coat_sigma = r.coat_sigma;
+    m.type = 0;
+    m.tex_mask = 0;
+    return m;
+}
+__device__ __noinline__ f3 rare_eval_v(int id, RareMat r, f3 wo, f3 wi) {
+    const DevMat m = rare_unpack(r);
+    return rare_eval_inl(id, m, wo, wi);
+}
+__device__ __noinline__ float rare_pdf_v(int id, RareMat r, f3 wo, f3 wi) {
+    const DevMat m = rare_unpack(r);
+    return rare_pdf_inl(id, m, wo, wi);
 }
 
+// LOCAL: m is a copy in registers (see rare_eval_v)
+template <bool LOCAL = false>
 __device__ __forceinline__ f3 lobe_eval(const Lobe& L, const Isect& I, const DevMat& m, f3 wo, f3 wi) {
     switch (L.id) {
         case L_LAMBERT_R:
@@ -1637,10 +1704,11 @@ __device__ __forceinline__ f3 lobe_eval(const Lobe& L, const Isect& I, const Dev
         case L_SHADOWCATCHER: return sdiv3(I.catcher, fabsf(wi.z));
         case L_MICRO_T:
         case L_FRESNEL_BLEND:
-        case L_CLEARCOAT: return rare_eval(L.id, &m, wo, wi);
+        case L_CLEARCOAT: return LOCAL ? rare_eval_v(L.id, rare_pack(m), wo, wi) : rare_eval(L.id, &m, wo, wi);
         default: return f3{0.0f, 0.0f, 0.0f};
     }
 }
+template <bool LOCAL = false>
 __device__ __forceinline__ float lobe_pdf(const Lobe& L, const DevMat& m, f3 wo, f3 wi) {
     switch (L.id) {
         case L_LAMBERT_R: return same_hemi(wo, wi) ? fabsf(wi.z) * K_INV_PI : 0.0f;
@@ -1650,7 +1718,7 @@ __device__ __forceinline__ float lobe_pdf(const Lobe& L, const DevMat& m, f3 wo,
         case L_SHADOWCATCHER: return 1.0f;
         case L_MICRO_T:
         case L_FRESNEL_BLEND:
-        case L_CLEARCOAT: return rare_pdf(L.id, &m, wo, wi);
+        case L_CLEARCOAT: return LOCAL ? rare_pdf_v(L.id, rare_pack(m), wo, wi) : rare_pdf(L.id, &m, wo, wi);
         default: return 0.0f;
     }
 }
@@ -1667,6 +1735,7 @@ __device__ __forceinline__ f3 micro_sample(int ltype, const DevMat& m, f3 wo, f3
     pdf = ggx_pdf(g, wo, wh) / (4.0f * dot3(wo, wh));
     return micro_eval(ltype, m, wo, wi);
 }
+template <bool LOCAL = false>
 __device__ __forceinline__ f3 lobe_sample(const Lobe& L, const Isect& I, const DevMat& m, f3 wo, f3& wi, f2 u,
                                           float& pdf, int& sampled) {
     switch (L.id) {
@@ -1679,8 +1748,8 @@ __device__ __forceinline__ f3 lobe_sample(const Lobe& L, const Isect& I, const D
             } else {
                 if (wo.z > 0.0f) wi.z *= -1.0f;
             }
-            pdf = lobe_pdf(L, m, wo, wi);
-            return lobe_eval(L, I, m, wo, wi);
+            pdf = lobe_pdf<LOCAL>(L, m, wo, wi);
+            return lobe_eval<LOCAL>(L, I, m, wo, wi);
         }
         case L_MICRO_R_DIEL:
         case L_MICRO_R_COND:
@@ -1799,6 +1868,7 @@ __device__ __forceinline__ f3 lobe_sample(const Lobe& L, const Isect& I, const D
 }
 
 // CudaBSDF::Eval / Pdf / Sample_f (noor/cuda/bsdf.cuh:259-356)
+template <bool LOCAL = false>
 __device__ __forceinline__ f3 bsdf_eval(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
     const f3 wo = to_local(I, woW), wi = to_local(I, wiW);
     const bool reflect = dot3(wiW, I.gn) * dot3(woW, I.gn) > 0.0f;
@@ -1806,9 +1876,10 @@ __device__ __forceinline__ f3 bsdf_eval(const Bsdf& b, const Isect& I, const Dev
     for (int i = 0; i < b.n; ++i)
         if (lobe_matches(b.l[i].type, flags) &&
             ((reflect && (b.l[i].type & B_REFLECTION)) || (!reflect && (b.l[i].type & B_TRANSMISSION))))
-            f = add3(f, lobe_eval(b.l[i], I, m, wo, wi));
+            f = add3(f, lobe_eval<LOCAL>(b.l[i], I, m, wo, wi));
     return f;
 }
+template <bool LOCAL = false>
 __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3 wiW, int flags) {
     if (b.n == 0) return 0.0f;
     const f3 wo = to_local(I, woW), wi = to_local(I, wiW);
@@ -1818,10 +1889,11 @@ __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, const Isect& I, const D
     for (int i = 0; i < b.n; ++i)
         if (lobe_matches(b.l[i].type, flags)) {
             ++matching;
-            pdf += lobe_pdf(b.l[i], m, wo, wi);
+            pdf += lobe_pdf<LOCAL>(b.l[i], m, wo, wi);
         }
     return matching > 0 ? pdf / (float)matching : 0.0f;
 }
+template <bool LOCAL = false>
 __device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const DevMat& m, f3 woW, f3& wiW, float& pdf,
                           int flags, int& sampled, Rng& rng, int rk, int* lobe_out = nullptr) {
     int matching = 0;
@@ -1852,7 +1924,7 @@ __device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const D
     pdf = 0.0f;
     sampled = L.type;
     f3 wi = f3{0.0f, 0.0f, 0.0f};
-    f3 f = lobe_sample(L, I, m, wo, wi, ur, pdf, sampled);
+    f3 f = lobe_sample<LOCAL>(L, I, m, wo, wi, ur, pdf, sampled);
     if (pdf == 0.0f) {
         sampled = 0;
         return f3{0.0f, 0.0f, 0.0f};
@@ -1861,14 +1933,14 @@ __device__ __forceinline__ f3 bsdf_sample(const Bsdf& b, const Isect& I, const D
     const bool spec = (L.type & B_SPECULAR) != 0;
     if (!spec && matching > 1)
         for (int i = 0; i < b.n; ++i)
-            if (i != li && lobe_matches(b.l[i].type, flags)) pdf += lobe_pdf(b.l[i], m, wo, wi);
+            if (i != li && lobe_matches(b.l[i].type, flags)) pdf += lobe_pdf<LOCAL>(b.l[i], m, wo, wi);
     if (matching > 1) pdf /= (float)matching;
     if (!spec) {
         const bool reflect = dot3(wiW, I.gn) * dot3(woW, I.gn) > 0.0f;
         for (int i = 0; i < b.n; ++i)
             if (i != li && lobe_matches(b.l[i].type, flags) &&
                 ((reflect && (b.l[i].type & B_REFLECTION)) || (!reflect && (b.l[i].type & B_TRANSMISSION))))
-                f = add3(f, lobe_eval(b.l[i], I, m, wo, wi));
+                f = add3(f, lobe_eval<LOCAL>(b.l[i], I, m, wo, wi));
     }
     return f;
 }
@@ -2379,7 +2451,10 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
     if (ps.bounce == 0 && lookat) *lookat = I.p;
     // the material with its image textures looked up (TEX), else the constants
     DevMat mt;
-    if (TEX && !(I.type & NOOR_MAT_MESHLIGHT)) resolve_material(S, S.mats + I.mat, I.u, I.v, hd.g, &mt);
+    if (TEX) {
+        mt = S.mats[I.mat];
+        if (!(I.type & NOOR_MAT_MESHLIGHT)) resolve_material(S, I.u, I.v, hd.g, mt);
+    }
     if (is_emitter(I) && (ps.bounce == 0 || ps.specular)) {
         const f3 Le = (I.type & NOOR_MAT_MESHLIGHT) ? light_Le(S, ray.d, (int)I.ins)
                                                     : ld3(TEX ? mt.emittance : S.mats[I.mat].emittance);
@@ -2387,10 +2462,15 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
         return false;
     }
     // accumulate: bump before the BSDF is built (noor/cuda/accumulate.cuh:84-99)
-    if (TEX && rd->has && (I.type & NOOR_MAT_BUMP)) bump_frame(S, S.mats[I.mat], hd, I);
+    if (TEX && rd->has && (I.type & NOOR_MAT_BUMP)) {
+        const Frame F = bump_frame(S, S.mats[I.mat], I.u, I.v, hd.g, Frame{I.ss, I.ts, I.sn}, I.gn);
+        I.ss = F.ss;
+        I.ts = F.ts;
+        I.sn = F.sn;
+    }
     Bsdf b;
     make_bsdf(I.type, b);
-    const DevMat& m = (TEX && !(I.type & NOOR_MAT_MESHLIGHT)) ? mt : S.mats[I.mat];
+    const DevMat& m = TEX ? mt : S.mats[I.mat];
     // sampleLight (noor/cuda/direct.cuh:33-67)
     nee.has = 1;
     nee.beta = ps.beta;
@@ -2405,14 +2485,14 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
         const f3 Li = light_sample_Li(S, I, light_idx, vis, light_pdf, ps.rng);
         if (!(light_pdf == 0.0f || is_black(Li))) {
             light_ok = true;
-            f3 f = bsdf_eval(b, I, m, I.wo, vis.wi, flags);
+            f3 f = bsdf_eval<TEX>(b, I, m, I.wo, vis.wi, flags);
             if (!is_black(f)) {
                 f = scl3(f, absdot3(vis.wi, I.sn));
                 f3 c;
                 if (is_shadowcatcher(I)) c = f;
                 else if (is_delta(S, light_idx) || !S.mis) c = sdiv3(mul3(Li, f), light_pdf);
                 else {
-                    const float sp = bsdf_pdf(b, I, m, I.wo, vis.wi, flags);
+                    const float sp = bsdf_pdf<TEX>(b, I, m, I.wo, vis.wi, flags);
                     const float w = power_heuristic(light_pdf, sp);
                     c = sdiv3(scl3(mul3(Li, f), w), light_pdf);
                 }
@@ -2440,7 +2520,7 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
         float spdf = 0.0f;
         int sampled = 0;
         f3 wi = f3{0.0f, 0.0f, 0.0f};
-        f3 f = bsdf_sample(b, I, m, I.wo, wi, spdf, flags, sampled, ps.rng, rk);
+        f3 f = bsdf_sample<TEX>(b, I, m, I.wo, wi, spdf, flags, sampled, ps.rng, rk);
         if (!(is_black(f) || spdf == 0.0f)) {
             f = scl3(f, absdot3(wi, I.sn));
             float sw = 1.0f;
@@ -2478,7 +2558,7 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
     int sampled = 0;
     int lobe = -1;
     f3 wi = I.wi;
-    const f3 f = bsdf_sample(b, I, m, I.wo, wi, pdf, B_ALL, sampled, ps.rng, rk, tp ? &lobe : nullptr);
+    const f3 f = bsdf_sample<TEX>(b, I, m, I.wo, wi, pdf, B_ALL, sampled, ps.rng, rk, tp ? &lobe : nullptr);
     I.wi = wi;
     if (tp) {
         tp->scat = (pdf != 0.0f) ? 1 : 0;

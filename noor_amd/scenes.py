@@ -523,14 +523,24 @@ def quad_uv(corner, u, v):
 
 
 def textured(seed=8, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, lens_radius=0.0,
-             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL, tex=64):
+             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL, tex=64, flat=False):
     """The image-texture pipeline of the reference: mip-mapped colour,
     roughness and emittance textures read at the pixel footprint (ray
     differentials from the camera, carried through mirror, metal and glass
     bounces), bump maps and alpha-cut surfaces, in the open Cornell geometry
-    under an HDR environment."""
+    under an HDR environment.  `flat` replaces every image by its mean and
+    drops the bump and alpha maps: the same scene without textures."""
     rng = np.random.default_rng(seed)
     sb = SceneBuilder()
+    mat = material
+    if flat:
+        def mat(mtype, bump=None, alpha=None, bumpfactor=1.0, uvscale=None, **kw):
+            for k, v in kw.items():
+                if isinstance(v, np.ndarray):
+                    a = v.reshape(-1, v.shape[-1]) if v.ndim == 3 else v.reshape(-1, 1)
+                    m = a.mean(0)
+                    kw[k] = tuple(float(x) for x in (m if m.size > 1 else np.repeat(m, 3)))
+            return material(mtype, **kw)
     ck = checker(tex, 8)
     bumpimg = noise_image(tex, seed + 1)
     stripes = np.zeros((tex // 2, tex // 2, 3), np.float32)
@@ -543,19 +553,19 @@ def textured(seed=8, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, len
     glow = np.zeros((16, 16, 3), np.float32)
     glow[:, :, :] = (np.arange(16)[:, None, None] % 4 < 2) * np.array([6.0, 5.0, 3.0], np.float32)
     mats = dict(
-        floor=sb.add_material(material(abi.MAT_DIFFUSE, diffuse=ck, bump=bumpimg, bumpfactor=0.02,
+        floor=sb.add_material(mat(abi.MAT_DIFFUSE, diffuse=ck, bump=bumpimg, bumpfactor=0.02,
                                        uvscale={"diffuse": (4, 4), "bump": (2, 2)})),
-        back=sb.add_material(material(abi.MAT_GLOSSY, diffuse=stripes, specular=(0.3, 0.3, 0.3), roughness=rough,
+        back=sb.add_material(mat(abi.MAT_GLOSSY, diffuse=stripes, specular=(0.3, 0.3, 0.3), roughness=rough,
                                       uvscale={"diffuse": (2, 1), "roughness": (1, 2)})),
-        left=sb.add_material(material(abi.MAT_METAL, eta=CU_ETA, k=CU_K, roughness=rough * 0.5)),
-        right=sb.add_material(material(abi.MAT_MIRROR, specular=(0.9, 0.9, 0.9))),
-        cut=sb.add_material(material(abi.MAT_DIFFUSE, diffuse=ck[:, :, ::-1].copy(), alpha=holes,
+        left=sb.add_material(mat(abi.MAT_METAL, eta=CU_ETA, k=CU_K, roughness=rough * 0.5)),
+        right=sb.add_material(mat(abi.MAT_MIRROR, specular=(0.9, 0.9, 0.9))),
+        cut=sb.add_material(mat(abi.MAT_DIFFUSE, diffuse=ck[:, :, ::-1].copy(), alpha=holes,
                                      uvscale={"transparency": (1, 1)})),
-        glass=sb.add_material(material(abi.MAT_GLASS, specular=(1, 1, 1), transmission=(1, 1, 1), coat_ior=1.5)),
-        torus=sb.add_material(material(abi.MAT_SUBSTRATE, diffuse=stripes, specular=(0.05, 0.05, 0.05),
+        glass=sb.add_material(mat(abi.MAT_GLASS, specular=(1, 1, 1), transmission=(1, 1, 1), coat_ior=1.5)),
+        torus=sb.add_material(mat(abi.MAT_SUBSTRATE, diffuse=stripes, specular=(0.05, 0.05, 0.05),
                                        roughness=(0.15, 0.15), bump=bumpimg, bumpfactor=0.01,
                                        uvscale={"diffuse": (4, 1), "bump": (8, 2)})),
-        glow=sb.add_material(material(abi.MAT_EMITTER, emittance=glow, emitter_scale=1.5)),
+        glow=sb.add_material(mat(abi.MAT_EMITTER, emittance=glow, emitter_scale=1.5)),
     )
     walls = [
         (quad_uv((-1.0, -0.5, -1.5), (2.0, 0.0, 0.0), (0.0, 0.0, 2.0)), mats["floor"]),    # floor, normal +y
