@@ -138,7 +138,13 @@ typedef struct noor_spec {
 
 /* CudaMaterial (noor/cuda/material.cuh:26-58), 128 bytes.  Every colour is a
  * texture index into noor_payload_view.textures; constants are 1x1 textures
- * (noor/source/model.cpp:211-299). */
+ * (noor/source/model.cpp:211-299).  Larger textures are image textures, read
+ * mip-mapped at the pixel footprint (transmission, ior and k at uv scale 1,
+ * roughness at level 0).  Type bit NOOR_MAT_BUMP reads bump_tex_idx (first
+ * channel, times bumpfactor, at bump_uvscale); NOOR_MAT_ALPHA reads
+ * transparency_tex_idx (first channel, point-sampled at transparency_uvscale;
+ * <= 1e-4 cuts the surface away).  The v coordinate is flipped (1 - v) on
+ * read, as the reference's mesh manager does. */
 typedef struct noor_material {
     noor_float2 diffuse_uvscale, specular_uvscale, transparency_uvscale,
                 emittance_uvscale, bump_uvscale, roughness_uvscale, metalness_uvscale;
@@ -173,9 +179,12 @@ typedef struct noor_spot_light {
 } noor_spot_light;
 typedef struct noor_distant_light { noor_float3 ke, power; int32_t type; noor_float3 direction; } noor_distant_light;
 
-/* ImageTexture as the device sees it: RGBA32F texels, row-major, clamp
- * addressing, bilinear filtering at level 0 (noor/cuda/texture.cuh:30-138).
- * The last texture of the payload is the environment / sky texture. */
+/* ImageTexture as the device sees it: RGBA32F texels, row-major, row 0 at
+ * v = 0.  Material image textures need power-of-two sides (at most 32768):
+ * wrap addressing, a mip chain as CudaMipMap builds it, bilinear filtering
+ * within a level and linear between levels (noor/cuda/texture.cuh:30-400).
+ * The last texture of the payload is the environment / sky texture (any
+ * size, clamp addressing, bilinear at level 0). */
 typedef struct noor_texture_view {
     uint32_t width, height;
     const float* rgba;      /* width*height*4 floats */
