@@ -246,7 +246,10 @@ enum { NOOR_TRI_WATERTIGHT = 0, NOOR_TRI_WALD = 1 };
 #define NOOR_COMPAT_MIS_SELF_SHADOW 0x20u /* MIS: the shadow ray to a BSDF-sampled light point ends on the light
                                              and can be blocked by it (noor/cuda/direct.cuh:98-104,
                                              noor/cuda/intersection.cuh:201-207) */
-#define NOOR_COMPAT_ALL           0x3Fu
+#define NOOR_COMPAT_RESIZE_SUM    0x40u /* non-power-of-two textures: the horizontal and vertical Lanczos passes
+                                         are summed, not composed (noor/cuda/image.cuh:61-123); read at
+                                         noor_create only */
+#define NOOR_COMPAT_ALL           0x7Fu
 
 typedef struct noor_options {
     int32_t rng;            /* NOOR_RNG_* */

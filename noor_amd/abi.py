@@ -242,8 +242,8 @@ CAMERA_PERSP, CAMERA_ORTHO, CAMERA_ENV = 0, 1, 2
 RNG_TEA_LCG, RNG_PCG32 = 0, 1
 TRI_WATERTIGHT, TRI_WALD = 0, 1
 COMPAT_QUAD_PDF, COMPAT_CDF_N_PLUS_1, COMPAT_CLAMP, COMPAT_WASTED_DRAWS = 1, 2, 4, 8
-COMPAT_MIS_LIGHT_COUNT, COMPAT_MIS_SELF_SHADOW = 16, 32
-COMPAT_ALL = 63
+COMPAT_MIS_LIGHT_COUNT, COMPAT_MIS_SELF_SHADOW, COMPAT_RESIZE_SUM = 16, 32, 64
+COMPAT_ALL = 127
 LIGHT_SKY_MASK = 0x20
 
 OK, ERR_INVALID_ARG, ERR_INVALID_PAYLOAD, ERR_HIP, ERR_OUT_OF_MEMORY = 0, 1, 2, 3, 4

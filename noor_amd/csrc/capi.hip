@@ -93,7 +93,6 @@ const uint32_t kSupportedMaterials = NOOR_MAT_DIFFUSE | NOOR_MAT_TRANSLUCENT | N
                                      NOOR_MAT_SUBSTRATE | NOOR_MAT_CLEARCOAT | NOOR_MAT_SHADOWCATCHER |
                                      NOOR_MAT_ALPHA | NOOR_MAT_BUMP;
 
-bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
 noor_status validate(Context* c, const noor_payload_view* P) {
     if (P->num_triangles && (!P->attribute_indices || !P->wald_ax_nu_nv_nd || !P->wald_bnu_bnv_au_av ||
@@ -155,9 +154,8 @@ noor_status validate(Context* c, const noor_payload_view* P) {
             const noor_texture_view& tv = P->textures[t];
             if (!tv.rgba || tv.width == 0 || tv.height == 0)
                 return fail(c, NOOR_ERR_INVALID_PAYLOAD, "empty material texture");
-            if (tv.width > 1u << 15 || tv.height > 1u << 15 || !is_pow2(tv.width) || !is_pow2(tv.height))
-                return fail(c, NOOR_ERR_UNSUPPORTED,
-                            "material textures must be powers of two in each dimension, at most 32768");
+            if (tv.width > 1u << 15 || tv.height > 1u << 15)
+                return fail(c, NOOR_ERR_UNSUPPORTED, "material textures are at most 32768 texels a side");
         }
         if ((mt.type & NOOR_MAT_BUMP) && !std::isfinite(mt.bumpfactor))
             return fail(c, NOOR_ERR_INVALID_PAYLOAD, "non-finite bump factor");
@@ -405,7 +403,9 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
         auto stored = [&](uint32_t t) -> uint32_t {
             if (store_idx[t] < 0) {
                 const noor_texture_view& tv = P->textures[t];
-                store_idx[t] = noor::store_add_texture(store, tv.rgba, tv.width, tv.height);
+                store_idx[t] = noor::store_add_texture(store, tv.rgba, tv.width, tv.height,
+                                                       ((options ? options->compat : NOOR_COMPAT_ALL) &
+                                                        NOOR_COMPAT_RESIZE_SUM) != 0);
             }
             return (uint32_t)store_idx[t];
         };

@@ -39,7 +39,102 @@ void level_bilinear(const float* lvl, uint32_t w, uint32_t h, float s, float q, 
         out[c] = by * (bx * t00[c] + ax * t10[c]) + ay * (bx * t01[c] + ax * t11[c]);
 }
 
-uint32_t store_add_texture(TextureStore& ts, const float* rgba, uint32_t w, uint32_t h) {
+uint32_t nearest_pow2(uint32_t n) {
+    uint32_t next = 1;
+    while (next < n) next <<= 1;
+    uint32_t prev = next > n ? next >> 1 : next;
+    if (prev < 1) prev = 1;
+    return (next - n < n - prev) ? next : prev;
+}
+
+// Lanczos windowed sinc with tau 2 (noor/cuda/image.cuh:41-50), evaluated in
+// binary64 and rounded.
+static float lanczos(float x) {
+    double a = std::fabs((double)x);
+    if (a < 1e-5) return 1.0f;
+    if (a > 1.0) return 0.0f;
+    a *= 3.14159265358979323846;
+    const double s = std::sin(a / 2.0) / (a / 2.0);
+    const double l = std::sin(a) / a;
+    return (float)(s * l);
+}
+
+// The four taps of one output texel: first source texel and normalised
+// weights (ResampleWeight, noor/cuda/image.cuh:52-98).
+static void taps(float center, int& first, float wt[4]) {
+    first = (int)std::floor(center - 2.0f + 0.5f);
+    float sum = 0.0f;
+    for (int j = 0; j < 4; ++j) {
+        const float pos = (float)(first + j) + 0.5f;
+        wt[j] = lanczos((pos - center) / 2.0f);
+        sum += wt[j];
+    }
+    for (int j = 0; j < 4; ++j) wt[j] /= sum;
+}
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+std::vector<float> lanczos_resize(const float* src, uint32_t w, uint32_t h, uint32_t W, uint32_t H,
+                                  bool resize_sum) {
+    std::vector<float> out(4 * (size_t)W * H, 0.0f);
+    const float xs = (float)w / (float)W, ys = (float)h / (float)H;
+    if (resize_sum) {
+        for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) {
+                const int cx = (int)(((float)x + 0.5f) * xs), cy = (int)(((float)y + 0.5f) * ys);
+                int fx, fy;
+                float wx[4], wy[4];
+                taps((float)cx, fx, wx);
+                taps((float)cy, fy, wy);
+                float* o = out.data() + 4 * ((size_t)y * W + x);
+                for (int j = 0; j < 4; ++j) {
+                    const float* t = src + 4 * ((size_t)cy * w + clampi(fx + j, 0, (int)w - 1));
+                    for (int c = 0; c < 4; ++c) o[c] += wx[j] * t[c];
+                }
+                for (int j = 0; j < 4; ++j) {
+                    const float* t = src + 4 * ((size_t)clampi(fy + j, 0, (int)h - 1) * w + cx);
+                    for (int c = 0; c < 4; ++c) o[c] += wy[j] * t[c];
+                }
+            }
+        return out;
+    }
+    std::vector<float> tmp(4 * (size_t)W * h, 0.0f);
+    for (uint32_t x = 0; x < W; ++x) {
+        int fx;
+        float wx[4];
+        taps(((float)x + 0.5f) * xs, fx, wx);
+        for (uint32_t r = 0; r < h; ++r) {
+            float* o = tmp.data() + 4 * ((size_t)r * W + x);
+            for (int j = 0; j < 4; ++j) {
+                const float* t = src + 4 * ((size_t)r * w + clampi(fx + j, 0, (int)w - 1));
+                for (int c = 0; c < 4; ++c) o[c] += wx[j] * t[c];
+            }
+        }
+    }
+    for (uint32_t y = 0; y < H; ++y) {
+        int fy;
+        float wy[4];
+        taps(((float)y + 0.5f) * ys, fy, wy);
+        for (uint32_t x = 0; x < W; ++x) {
+            float* o = out.data() + 4 * ((size_t)y * W + x);
+            for (int j = 0; j < 4; ++j) {
+                const float* t = tmp.data() + 4 * ((size_t)clampi(fy + j, 0, (int)h - 1) * W + x);
+                for (int c = 0; c < 4; ++c) o[c] += wy[j] * t[c];
+            }
+        }
+    }
+    return out;
+}
+
+uint32_t store_add_texture(TextureStore& ts, const float* rgba, uint32_t w, uint32_t h, bool resize_sum) {
+    const uint32_t W = nearest_pow2(w), H = nearest_pow2(h);
+    std::vector<float> resized;
+    if (W != w || H != h) {
+        resized = lanczos_resize(rgba, w, h, W, H, resize_sum);
+        rgba = resized.data();
+        w = W;
+        h = H;
+    }
     TexLevels d;
     d.offset = (uint32_t)(ts.texels.size() / 4);
     d.w = w;

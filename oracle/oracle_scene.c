@@ -284,11 +284,116 @@ static void level_bilinear(const float* lvl, uint32_t w, uint32_t h, float s, fl
         out[c] = by * (bx * t00[c] + ax * t10[c]) + ay * (bx * t01[c] + ax * t11[c]);
 }
 
+/* The power of two nearest n, the smaller on a tie (NOOR::nearestPow2,
+ * noor/cuda/utils.cuh:129-135). */
+static uint32_t nearest_pow2u(uint32_t n) {
+    uint32_t next = 1;
+    while (next < n) next <<= 1;
+    uint32_t prev = next > n ? next >> 1 : next;
+    if (prev < 1) prev = 1;
+    return (next - n < n - prev) ? next : prev;
+}
+
+/* Lanczos windowed sinc, tau 2 (noor/cuda/image.cuh:41-50), in binary64, rounded */
+static float lanczos2(float x) {
+    double a = fabs((double)x);
+    if (a < 1e-5) return 1.0f;
+    if (a > 1.0) return 0.0f;
+    a *= 3.14159265358979323846;
+    const double s = sin(a / 2.0) / (a / 2.0);
+    const double l = sin(a) / a;
+    return (float)(s * l);
+}
+
+/* ResampleWeight (noor/cuda/image.cuh:52-98): first tap, normalised weights */
+static void resample_taps(float center, int* first, float wt[4]) {
+    *first = (int)floorf(center - 2.0f + 0.5f);
+    float sum = 0.0f;
+    for (int j = 0; j < 4; ++j) {
+        const float pos = (float)(*first + j) + 0.5f;
+        wt[j] = lanczos2((pos - center) / 2.0f);
+        sum += wt[j];
+    }
+    for (int j = 0; j < 4; ++j) wt[j] /= sum;
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* resize_kernel (noor/cuda/image.cuh:61-123), clamp addressing.  `sum`: as
+ * the reference, the horizontal taps along the source row and the vertical
+ * taps along the source column under each output texel (integer centres) are
+ * added; otherwise the two passes are composed, x first, with centres
+ * (i + 1/2) * old / new. */
+static float* lanczos_resize(const float* src, uint32_t w, uint32_t h, uint32_t W, uint32_t H, int sum) {
+    float* out = (float*)calloc(4 * (size_t)W * H, sizeof(float));
+    const float xs = (float)w / (float)W, ys = (float)h / (float)H;
+    if (sum) {
+        for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) {
+                const int cx = (int)(((float)x + 0.5f) * xs), cy = (int)(((float)y + 0.5f) * ys);
+                int fx, fy;
+                float wx[4], wy[4];
+                resample_taps((float)cx, &fx, wx);
+                resample_taps((float)cy, &fy, wy);
+                float* o = out + 4 * ((size_t)y * W + x);
+                for (int j = 0; j < 4; ++j) {
+                    const float* t = src + 4 * ((size_t)cy * w + clampi(fx + j, 0, (int)w - 1));
+                    for (int c = 0; c < 4; ++c) o[c] += wx[j] * t[c];
+                }
+                for (int j = 0; j < 4; ++j) {
+                    const float* t = src + 4 * ((size_t)clampi(fy + j, 0, (int)h - 1) * w + cx);
+                    for (int c = 0; c < 4; ++c) o[c] += wy[j] * t[c];
+                }
+            }
+        return out;
+    }
+    float* tmp = (float*)calloc(4 * (size_t)W * h, sizeof(float));
+    for (uint32_t x = 0; x < W; ++x) {
+        int fx;
+        float wx[4];
+        resample_taps(((float)x + 0.5f) * xs, &fx, wx);
+        for (uint32_t r = 0; r < h; ++r) {
+            float* o = tmp + 4 * ((size_t)r * W + x);
+            for (int j = 0; j < 4; ++j) {
+                const float* t = src + 4 * ((size_t)r * w + clampi(fx + j, 0, (int)w - 1));
+                for (int c = 0; c < 4; ++c) o[c] += wx[j] * t[c];
+            }
+        }
+    }
+    for (uint32_t y = 0; y < H; ++y) {
+        int fy;
+        float wy[4];
+        resample_taps(((float)y + 0.5f) * ys, &fy, wy);
+        for (uint32_t x = 0; x < W; ++x) {
+            float* o = out + 4 * ((size_t)y * W + x);
+            for (int j = 0; j < 4; ++j) {
+                const float* t = tmp + 4 * ((size_t)clampi(fy + j, 0, (int)h - 1) * W + x);
+                for (int c = 0; c < 4; ++c) o[c] += wy[j] * t[c];
+            }
+        }
+    }
+    free(tmp);
+    return out;
+}
+
 /* Appends a texture and its mip chain: each texel of the next level is the
  * mean of the four samples of this level at ((x + i) / W, (y + j) / H),
  * i, j in {0, 1}, (W, H) the next level's size (the mipmap kernel,
  * noor/cuda/texture.cuh:204-221, driven by generateMipMaps, 322-388). */
-static uint32_t store_add(orc_scene* s, const noor_texture_view* t, size_t* cap) {
+static uint32_t store_add(orc_scene* s, const noor_texture_view* t0, size_t* cap) {
+    /* sides resampled to the nearest powers of two first (CudaTexture::load,
+     * noor/cuda/texture.cuh:141-165) */
+    noor_texture_view tv = *t0;
+    const noor_texture_view* t = &tv;
+    float* resized = NULL;
+    const uint32_t W = nearest_pow2u(t0->width), H = nearest_pow2u(t0->height);
+    if (W != t0->width || H != t0->height) {
+        resized = lanczos_resize(t0->rgba, t0->width, t0->height, W, H,
+                                 (s->opt.compat & NOOR_COMPAT_RESIZE_SUM) != 0);
+        tv.width = W;
+        tv.height = H;
+        tv.rgba = resized;
+    }
     const uint32_t levels = mip_levels(t->width, t->height);
     size_t need = 0;
     uint32_t w = t->width, h = t->height;
@@ -338,6 +443,7 @@ static uint32_t store_add(orc_scene* s, const noor_texture_view* t, size_t* cap)
         sh = H;
     }
     s->tex_desc[s->num_tex] = d;
+    free(resized);
     return s->num_tex++;
 }
 

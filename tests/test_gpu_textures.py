@@ -37,6 +37,22 @@ def test_textured_megakernel_and_wavefront_vs_oracle(lens, mis, tri):
         np.testing.assert_array_equal(w.film(), g)
 
 
+@pytest.mark.parametrize("compat", [abi.COMPAT_ALL, 0])
+def test_resized_textures_vs_oracle(compat):
+    """48x48 and 24x24 images, resampled to 32x32 and 16x16 at creation (the
+    reference's summed Lanczos passes under compat, composed passes without):
+    the device store and the oracle's agree, so the films do."""
+    wl = scenes.textured(tex=48, compat=compat)
+    r = Renderer.from_workload(wl)
+    r.render(spp=2)
+    g = r.film()
+    c, _ = OracleScene.from_workload(wl).render(1, 2)
+    err = mean_rel_err(g, c)
+    print(f"compat={compat}: megakernel vs oracle {err:.3e}, identical {np.mean(np.all(g == c, axis=-1)):.4f}",
+          flush=True)
+    assert err <= TOL_IMAGE
+
+
 def test_alpha_cut_hit_queries_bit_exact():
     """Closest and any hit through the alpha-cut panel of the textured scene."""
     wl = scenes.textured()

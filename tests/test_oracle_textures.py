@@ -281,11 +281,66 @@ def test_textured_workload_renders_and_textures_matter():
     assert np.isfinite(f).all() and f[..., :3].mean() > 0.05
 
 
+def _lanczos_taps(center):
+    """ResampleWeight in float64 (noor/cuda/image.cuh:41-98): first tap and the
+    four normalised Lanczos (radius 2, tau 2) weights."""
+    first = int(np.floor(center - 2.0 + 0.5))
+    def L(x):
+        x = abs(x)
+        if x < 1e-5:
+            return 1.0
+        if x > 1:
+            return 0.0
+        x *= np.pi
+        return (np.sin(x / 2) / (x / 2)) * (np.sin(x) / x)
+    w = np.array([L((first + j + 0.5 - center) / 2.0) for j in range(4)])
+    return first, w / w.sum()
+
+
+@pytest.mark.parametrize("summed", [True, False])
+def test_lanczos_resize_of_non_power_of_two_textures(summed):
+    """Sides go to the nearest power of two (ties to the smaller): 12 -> 8,
+    40 -> 32, 48 -> 32, 100 -> 128.  With the reference's summed passes
+    (compat RESIZE_SUM) a constant image doubles and, at integer centres c,
+    f(x) = x^2 along x gives (c - 1/2)^2 + 1/4 + 4 w0 + c^2 with w0 the outer
+    tap weight (about 0.1058); composed, a constant stays and a texel equals
+    the separable float64 filter."""
+    h, w = 12, 100
+    compat = abi.COMPAT_ALL if summed else 0
+    const = np.full((h, w, 4), 0.3, np.float32)
+    wl = _plane_scene(material(abi.MAT_DIFFUSE, diffuse=const))
+    wl.options.compat = compat
+    lv = OracleScene.from_workload(wl).texture_levels(0)
+    assert lv[0].shape == (8, 128, 4)
+    np.testing.assert_allclose(lv[0], 0.6 if summed else 0.3, rtol=2e-6)
+    x = np.arange(w, dtype=np.float64)
+    quad = np.broadcast_to((x * x)[None, :, None], (h, w, 4)).astype(np.float32)
+    wl = _plane_scene(material(abi.MAT_DIFFUSE, diffuse=quad))
+    wl.options.compat = compat
+    got = OracleScene.from_workload(wl).texture_levels(0)[0].astype(np.float64)
+    W, H = 128, 8
+    _, wt = _lanczos_taps(10.0)
+    w0 = wt[0]
+    assert 0.10 < w0 < 0.11
+    for X in (5, 40, 77, 120):
+        for Y in (0, 3, 7):
+            if summed:
+                c = int((X + 0.5) * (w / W))
+                if c - 2 < 0 or c + 1 > w - 1:
+                    continue
+                expect = (c - 0.5) ** 2 + 0.25 + 4 * w0 + c * c
+            else:
+                first, wx = _lanczos_taps((X + 0.5) * (w / W))
+                xs = np.clip(first + np.arange(4), 0, w - 1).astype(np.float64)
+                expect = float(np.dot(wx, xs * xs))  # constant along y: the vertical pass keeps it
+            np.testing.assert_allclose(got[Y, X, 0], expect, rtol=2e-5)
+
+
 def test_payload_texture_validation():
-    """Material textures must have power-of-two sides (the reference resamples
-    others); the check runs before any device call."""
+    """Material textures are at most 32768 texels a side; the check runs
+    before any device call."""
     from noor_amd.renderer import Renderer
-    wl = _plane_scene(material(abi.MAT_DIFFUSE, diffuse=np.zeros((3, 4, 3), np.float32)))
+    wl = _plane_scene(material(abi.MAT_DIFFUSE, diffuse=np.zeros((2, 32769, 3), np.float32)))
     with pytest.raises(abi.NoorError) as e:
         Renderer.from_workload(wl)
     assert e.value.status == abi.ERR_UNSUPPORTED
