@@ -180,9 +180,11 @@ typedef struct noor_spot_light {
 typedef struct noor_distant_light { noor_float3 ke, power; int32_t type; noor_float3 direction; } noor_distant_light;
 
 /* ImageTexture as the device sees it: RGBA32F texels, row-major, row 0 at
- * v = 0.  Material image textures need power-of-two sides (at most 32768):
- * wrap addressing, a mip chain as CudaMipMap builds it, bilinear filtering
- * within a level and linear between levels (noor/cuda/texture.cuh:30-400).
+ * v = 0.  Material image textures (at most 32768 a side) are resampled to
+ * the nearest power of two per side (Lanczos, see NOOR_COMPAT_RESIZE_SUM),
+ * then read with wrap addressing from a mip chain as CudaMipMap builds it,
+ * bilinear within a level and linear between levels
+ * (noor/cuda/texture.cuh:30-400, noor/cuda/image.cuh:28-125).
  * The last texture of the payload is the environment / sky texture (any
  * size, clamp addressing, bilinear at level 0). */
 typedef struct noor_texture_view {
