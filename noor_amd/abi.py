@@ -229,6 +229,29 @@ class BuildStats(C.Structure):
     ]
 
 
+# include/noor_loaders.h
+PATH_MAX_ = 256
+
+
+class ObjMaterial(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("type", C.c_uint32), ("diffuse", C.c_float * 3),
+                ("specular", C.c_float * 3), ("transmission", C.c_float * 3), ("emittance", C.c_float * 3),
+                ("ior", C.c_float), ("roughness", C.c_float), ("transparency", C.c_float),
+                ("bumpfactor", C.c_float), ("uvscale", C.c_float * 2),
+                ("map_kd", C.c_char * PATH_MAX_), ("map_ks", C.c_char * PATH_MAX_), ("map_ke", C.c_char * PATH_MAX_),
+                ("map_d", C.c_char * PATH_MAX_), ("map_bump", C.c_char * PATH_MAX_)]
+
+
+class ObjView(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
+                ("num_triangles", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("material", C.POINTER(C.c_uint32)), ("num_materials", C.c_uint32),
+                ("materials", C.POINTER(ObjMaterial)), ("num_lights", C.c_uint32),
+                ("lights", C.POINTER(C.c_float)), ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
+                ("unitize_scale", C.c_float)]
+
+
 # enums (include/noor_hip.h)
 MAT_DIFFUSE, MAT_TRANSLUCENT, MAT_GLOSSY, MAT_MIRROR = 1, 4, 8, 16
 MAT_GLASS, MAT_ROUGHGLASS, MAT_METAL, MAT_EMITTER = 32, 64, 128, 512

@@ -18,7 +18,7 @@ BUILD = os.path.join(ROOT, "build", "noor_amd")
 
 HIP_SOURCES = ["capi.hip", "kernels_mega.hip", "kernels_env.hip", "kernels_wave.hip", "kernels_adjoint.hip",
                "bvh_build.hip"]
-CXX_SOURCES = ["scene_build.cpp", "wide_bvh.cpp", "texture_store.cpp"]
+CXX_SOURCES = ["scene_build.cpp", "wide_bvh.cpp", "texture_store.cpp", "loaders.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -19,6 +19,7 @@
 #include "ctx.h"
 #include "texture_store.h"
 #include "wide_bvh.h"
+#include "noor_loaders.h"
 #include "noor_scene.h"
 
 using noor::Context;
@@ -931,6 +932,8 @@ size_t noor_abi_struct_size(const char* name) {
     if (n == "noor_texture_view") return sizeof(noor_texture_view);
     if (n == "noor_kernel_time") return sizeof(noor_kernel_time);
     if (n == "noor_mesh_desc") return sizeof(noor_mesh_desc);
+    if (n == "noor_obj_material") return sizeof(noor_obj_material);
+    if (n == "noor_obj_view") return sizeof(noor_obj_view);
     if (n == "noor_instance_desc") return sizeof(noor_instance_desc);
     if (n == "noor_bvh_params") return sizeof(noor_bvh_params);
     if (n == "noor_scene_desc") return sizeof(noor_scene_desc);

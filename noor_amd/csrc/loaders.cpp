@@ -1,0 +1,617 @@
+// OBJ + MTL models and float images for the payload (include/noor_loaders.h).
+//
+// The reference reads OBJ through glm_obj (noor/source/objLoader.cpp:28-54)
+// and images through stb_image / tinyexr (noor/source/texture.cpp:49-76);
+// neither is shipped with it.  This file restates glm_obj's published
+// behaviour (fan triangulation, glmUnitize, glmFacetNormals,
+// glmVertexNormals with a crease angle) and reads Radiance HDR, PFM and PNM.
+#include "noor_loaders.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+struct noor_obj {
+    std::vector<float> pos, nrm, uv;
+    std::vector<uint32_t> idx, mat;
+    std::vector<noor_obj_material> mats;
+    std::vector<float> lights;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    float scale = 1.0f;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+noor_status fail(noor_status s, const std::string& m) {
+    g_err = m;
+    return s;
+}
+
+std::string dir_of(const std::string& p) {
+    const size_t k = p.find_last_of('/');
+    return k == std::string::npos ? std::string() : p.substr(0, k + 1);
+}
+
+void copy_str(char* dst, size_t n, const std::string& s) {
+    std::strncpy(dst, s.c_str(), n - 1);
+    dst[n - 1] = 0;
+}
+
+noor_obj_material default_material(const std::string& name) {
+    noor_obj_material m;
+    std::memset(&m, 0, sizeof(m));
+    copy_str(m.name, sizeof(m.name), name);
+    m.type = NOOR_MAT_DIFFUSE;
+    for (int c = 0; c < 3; ++c) {
+        m.diffuse[c] = 0.8f;
+        m.specular[c] = 0.0f;
+        m.transmission[c] = 0.0f;
+        m.emittance[c] = 0.0f;
+    }
+    m.ior = 1.0f;
+    m.roughness = 1.0f / 65.0f;  // glm's default shininess
+    m.transparency = 1.0f;
+    m.bumpfactor = 1.0f;
+    m.uvscale[0] = m.uvscale[1] = 1.0f;
+    return m;
+}
+
+uint32_t illum_type(int illum) {
+    switch (illum) {
+        case 1: return NOOR_MAT_GLOSSY;
+        case 3: return NOOR_MAT_MIRROR;
+        case 5: return NOOR_MAT_EMITTER;
+        case 6: return NOOR_MAT_METAL;
+        case 7: return NOOR_MAT_GLASS;
+        default: return NOOR_MAT_DIFFUSE;
+    }
+}
+
+// "map_Kd -s 2 2 -bm 0.5 file.png": options, then the file name (rest of line)
+void parse_map(std::istringstream& ls, std::string& file, float* uvscale, float* bm) {
+    std::string tok;
+    while (ls >> tok) {
+        if (tok == "-s") {
+            float a = 1, b = 1, c = 1;
+            ls >> a >> b;
+            // an optional third component
+            const std::streampos p = ls.tellg();
+            if (!(ls >> c)) {
+                ls.clear();
+                ls.seekg(p);
+            }
+            if (uvscale) {
+                uvscale[0] = a;
+                uvscale[1] = b;
+            }
+        } else if (tok == "-bm") {
+            float f = 1;
+            ls >> f;
+            if (bm) *bm = f;
+        } else if (tok == "-o" || tok == "-t") {
+            float a, b;
+            ls >> a >> b;
+        } else if (tok.size() > 1 && tok[0] == '-') {
+            float a;
+            ls >> a;
+        } else {
+            std::string rest;
+            std::getline(ls, rest);
+            file = tok + rest;
+            while (!file.empty() && (file.back() == ' ' || file.back() == '\r' || file.back() == '\t')) file.pop_back();
+            return;
+        }
+    }
+}
+
+noor_status read_mtl(const std::string& path, std::vector<noor_obj_material>& mats,
+                     std::map<std::string, uint32_t>& by_name) {
+    std::ifstream f(path);
+    if (!f) return fail(NOOR_ERR_INVALID_ARG, "cannot open material library " + path);
+    std::string line;
+    noor_obj_material* m = nullptr;
+    while (std::getline(f, line)) {
+        std::istringstream ls(line);
+        std::string k;
+        if (!(ls >> k) || k[0] == '#') continue;
+        if (k == "newmtl") {
+            std::string name;
+            ls >> name;
+            by_name[name] = (uint32_t)mats.size();
+            mats.push_back(default_material(name));
+            m = &mats.back();
+            continue;
+        }
+        if (!m) continue;
+        if (k == "Kd") ls >> m->diffuse[0] >> m->diffuse[1] >> m->diffuse[2];
+        else if (k == "Ks") ls >> m->specular[0] >> m->specular[1] >> m->specular[2];
+        else if (k == "Ke") ls >> m->emittance[0] >> m->emittance[1] >> m->emittance[2];
+        else if (k == "Tf") ls >> m->transmission[0] >> m->transmission[1] >> m->transmission[2];
+        else if (k == "Ni") ls >> m->ior;
+        else if (k == "Ns") {
+            float ns = 0;
+            ls >> ns;
+            m->roughness = 1.0f / ns;
+        } else if (k == "d") ls >> m->transparency;
+        else if (k == "Tr") {
+            float tr = 0;
+            ls >> tr;
+            m->transparency = 1.0f - tr;
+        } else if (k == "illum") {
+            int il = 0;
+            ls >> il;
+            m->type = illum_type(il);
+        } else {
+            std::string file;
+            if (k == "map_Kd") {
+                parse_map(ls, file, m->uvscale, nullptr);
+                copy_str(m->map_kd, NOOR_PATH_MAX, file);
+            } else if (k == "map_Ks") {
+                parse_map(ls, file, nullptr, nullptr);
+                copy_str(m->map_ks, NOOR_PATH_MAX, file);
+            } else if (k == "map_Ke") {
+                parse_map(ls, file, nullptr, nullptr);
+                copy_str(m->map_ke, NOOR_PATH_MAX, file);
+            } else if (k == "map_d") {
+                parse_map(ls, file, nullptr, nullptr);
+                copy_str(m->map_d, NOOR_PATH_MAX, file);
+            } else if (k == "map_bump" || k == "bump" || k == "map_Bump") {
+                parse_map(ls, file, nullptr, &m->bumpfactor);
+                copy_str(m->map_bump, NOOR_PATH_MAX, file);
+            }
+        }
+    }
+    return NOOR_OK;
+}
+
+// OBJ index: 1-based, negative counts back from the end; 0 = absent
+bool obj_index(const std::string& s, size_t n, uint32_t& out) {
+    if (s.empty()) return false;
+    char* end = nullptr;
+    const long v = std::strtol(s.c_str(), &end, 10);
+    if (end == s.c_str()) return false;
+    long i = v > 0 ? v - 1 : (long)n + v;
+    if (v == 0 || i < 0 || i >= (long)n) return false;
+    out = (uint32_t)i;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* noor_loader_last_error(void) { return g_err.c_str(); }
+
+noor_status noor_obj_load(const char* path, int unitize, float crease_deg, noor_obj** out) {
+    if (!path || !out) return NOOR_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::ifstream f(path);
+    if (!f) return fail(NOOR_ERR_INVALID_ARG, std::string("cannot open ") + path);
+    const std::string dir = dir_of(path);
+    std::vector<float> V, N, T;                        // file arrays
+    struct Corner { uint32_t v, t; bool has_t; };
+    std::vector<Corner> corners;                       // 3 per triangle
+    std::vector<uint32_t> tri_mat;
+    struct Group { std::string name; uint32_t first_tri, count; };
+    std::vector<Group> groups;
+    std::vector<noor_obj_material> mats;
+    std::map<std::string, uint32_t> by_name;
+    uint32_t cur_mat = UINT32_MAX;
+    std::string line;
+    size_t lineno = 0;
+    while (std::getline(f, line)) {
+        ++lineno;
+        std::istringstream ls(line);
+        std::string k;
+        if (!(ls >> k) || k[0] == '#') continue;
+        if (k == "v") {
+            float x = 0, y = 0, z = 0;
+            ls >> x >> y >> z;
+            if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
+                return fail(NOOR_ERR_INVALID_PAYLOAD, "non-finite vertex on line " + std::to_string(lineno));
+            V.insert(V.end(), {x, y, z});
+        } else if (k == "vn") {
+            float x = 0, y = 0, z = 0;
+            ls >> x >> y >> z;
+            N.insert(N.end(), {x, y, z});
+        } else if (k == "vt") {
+            float u = 0, v = 0;
+            ls >> u >> v;
+            T.insert(T.end(), {u, v});
+        } else if (k == "f") {
+            std::vector<Corner> poly;
+            std::string tok;
+            while (ls >> tok) {
+                std::string a[3];
+                int part = 0;
+                for (char ch : tok) {
+                    if (ch == '/') {
+                        if (++part > 2) break;
+                    } else {
+                        a[part] += ch;
+                    }
+                }
+                Corner c{0, 0, false};
+                if (!obj_index(a[0], V.size() / 3, c.v))
+                    return fail(NOOR_ERR_INVALID_PAYLOAD, "bad vertex index on line " + std::to_string(lineno));
+                c.has_t = obj_index(a[1], T.size() / 2, c.t);
+                poly.push_back(c);
+            }
+            if (poly.size() < 3) continue;
+            for (size_t i = 1; i + 1 < poly.size(); ++i) {
+                corners.push_back(poly[0]);
+                corners.push_back(poly[i]);
+                corners.push_back(poly[i + 1]);
+                tri_mat.push_back(cur_mat);
+                if (!groups.empty()) groups.back().count++;
+            }
+        } else if (k == "g" || k == "o") {
+            std::string name, rest;
+            ls >> name;
+            groups.push_back(Group{name, (uint32_t)tri_mat.size(), 0});
+        } else if (k == "usemtl") {
+            std::string name;
+            ls >> name;
+            auto it = by_name.find(name);
+            cur_mat = it == by_name.end() ? UINT32_MAX : it->second;
+        } else if (k == "mtllib") {
+            std::string name;
+            while (ls >> name) {
+                const noor_status s = read_mtl(dir + name, mats, by_name);
+                if (s != NOOR_OK) return s;
+            }
+        }
+    }
+    const uint32_t nt = (uint32_t)tri_mat.size();
+    if (nt == 0) return fail(NOOR_ERR_INVALID_PAYLOAD, std::string("no triangles in ") + path);
+    noor_obj* o = new noor_obj();
+    // default material for triangles without one
+    const uint32_t def = (uint32_t)mats.size();
+    bool need_def = false;
+    for (auto& m : tri_mat)
+        if (m == UINT32_MAX) {
+            m = def;
+            need_def = true;
+        }
+    if (need_def) mats.push_back(default_material("default"));
+    // glmUnitize
+    const size_t nv = V.size() / 3;
+    float mn[3] = {V[0], V[1], V[2]}, mx[3] = {V[0], V[1], V[2]};
+    for (size_t i = 1; i < nv; ++i)
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = std::min(mn[c], V[3 * i + c]);
+            mx[c] = std::max(mx[c], V[3 * i + c]);
+        }
+    for (int c = 0; c < 3; ++c) {
+        o->bmin[c] = mn[c];
+        o->bmax[c] = mx[c];
+    }
+    if (unitize) {
+        const float w = std::fabs(mx[0]) + std::fabs(mn[0]);
+        const float h = std::fabs(mx[1]) + std::fabs(mn[1]);
+        const float d = std::fabs(mx[2]) + std::fabs(mn[2]);
+        const float cx = (mx[0] + mn[0]) / 2.0f, cy = (mx[1] + mn[1]) / 2.0f, cz = (mx[2] + mn[2]) / 2.0f;
+        const float big = std::max(std::max(w, h), d);
+        const float scale = big > 0.0f ? 2.0f / big : 1.0f;
+        for (size_t i = 0; i < nv; ++i) {
+            V[3 * i + 0] = (V[3 * i + 0] - cx) * scale;
+            V[3 * i + 1] = (V[3 * i + 1] - cy) * scale;
+            V[3 * i + 2] = (V[3 * i + 2] - cz) * scale;
+        }
+        o->scale = scale;
+    }
+    // glmFacetNormals
+    std::vector<float> fn(3 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        const float* a = &V[3 * corners[3 * t].v];
+        const float* b = &V[3 * corners[3 * t + 1].v];
+        const float* c = &V[3 * corners[3 * t + 2].v];
+        const float u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const float v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        float n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const float l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) fn[3 * t + k] = l > 0.0f ? n[k] / l : 0.0f;
+    }
+    // glmVertexNormals: per vertex, the triangles that use it; the reference
+    // triangle is the last one in file order (the head of glm's prepend list)
+    std::vector<std::vector<uint32_t>> users(nv);
+    for (uint32_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) users[corners[3 * t + k].v].push_back(t);
+    const float cos_angle = std::cos(crease_deg * 3.14159265358979323846f / 180.0f);
+    std::vector<float> cn(9 * (size_t)nt);   // per-corner normal
+    for (size_t i = 0; i < nv; ++i) {
+        const auto& U = users[i];
+        if (U.empty()) continue;
+        const float* ref = &fn[3 * U.back()];
+        float avg[3] = {0, 0, 0};
+        std::vector<bool> within(U.size());
+        for (size_t j = U.size(); j-- > 0;) {  // glm walks its list from the last triangle
+            const float* q = &fn[3 * U[j]];
+            within[j] = q[0] * ref[0] + q[1] * ref[1] + q[2] * ref[2] > cos_angle;
+            if (within[j])
+                for (int k = 0; k < 3; ++k) avg[k] += q[k];
+        }
+        const float l = std::sqrt(avg[0] * avg[0] + avg[1] * avg[1] + avg[2] * avg[2]);
+        for (int k = 0; k < 3; ++k) avg[k] = l > 0.0f ? avg[k] / l : 0.0f;
+        for (size_t j = 0; j < U.size(); ++j) {
+            const uint32_t t = U[j];
+            for (int c = 0; c < 3; ++c) {
+                if (corners[3 * t + c].v != i) continue;
+                for (int k = 0; k < 3; ++k) cn[9 * (size_t)t + 3 * c + k] = within[j] ? avg[k] : fn[3 * t + k];
+            }
+        }
+    }
+    // unified vertices: one per distinct (position, normal, texcoord) of a corner
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, uint32_t> uniq;
+    for (uint32_t t = 0; t < nt; ++t)
+        for (int c = 0; c < 3; ++c) {
+            const Corner& k = corners[3 * t + c];
+            const float* n = &cn[9 * (size_t)t + 3 * c];
+            uint32_t nb[3];
+            std::memcpy(nb, n, sizeof(nb));
+            const uint32_t tix = k.has_t ? k.t : UINT32_MAX;
+            const auto key = std::make_tuple(k.v, tix, nb[0], nb[1], nb[2]);
+            auto it = uniq.find(key);
+            uint32_t id;
+            if (it == uniq.end()) {
+                id = (uint32_t)(o->pos.size() / 3);
+                uniq.emplace(key, id);
+                o->pos.insert(o->pos.end(), {V[3 * k.v], V[3 * k.v + 1], V[3 * k.v + 2]});
+                o->nrm.insert(o->nrm.end(), {n[0], n[1], n[2]});
+                if (k.has_t) o->uv.insert(o->uv.end(), {T[2 * k.t], T[2 * k.t + 1]});
+                else o->uv.insert(o->uv.end(), {0.0f, 0.0f});
+            } else {
+                id = it->second;
+            }
+            o->idx.push_back(id);
+        }
+    o->mat = tri_mat;
+    o->mats = mats;
+    // light groups (noor/source/objLoader.cpp:56-79): a quad from the first triangle
+    for (const Group& g : groups) {
+        if (g.name.find("light") == std::string::npos || g.count == 0) continue;
+        const uint32_t t = g.first_tri;
+        const float* base = &V[3 * corners[3 * t + 1].v];
+        const float* v1 = &V[3 * corners[3 * t + 2].v];
+        const float* v2 = &V[3 * corners[3 * t + 0].v];
+        const float u[3] = {v1[0] - base[0], v1[1] - base[1], v1[2] - base[2]};
+        const float v[3] = {v2[0] - base[0], v2[1] - base[1], v2[2] - base[2]};
+        float n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const float l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) n[k] = l > 0.0f ? n[k] / l : 0.0f;
+        const float* ke = mats[tri_mat[t]].emittance;
+        o->lights.insert(o->lights.end(), {base[0], base[1], base[2], u[0], u[1], u[2], v[0], v[1], v[2], n[0], n[1],
+                                           n[2], ke[0], ke[1], ke[2]});
+    }
+    *out = o;
+    return NOOR_OK;
+}
+
+noor_status noor_obj_view_get(const noor_obj* o, noor_obj_view* v) {
+    if (!o || !v) return NOOR_ERR_INVALID_ARG;
+    std::memset(v, 0, sizeof(*v));
+    v->num_vertices = (uint32_t)(o->pos.size() / 3);
+    v->positions = o->pos.data();
+    v->normals = o->nrm.data();
+    v->uvs = o->uv.data();
+    v->num_triangles = (uint32_t)(o->idx.size() / 3);
+    v->indices = o->idx.data();
+    v->material = o->mat.data();
+    v->num_materials = (uint32_t)o->mats.size();
+    v->materials = o->mats.data();
+    v->num_lights = (uint32_t)(o->lights.size() / 15);
+    v->lights = o->lights.data();
+    for (int c = 0; c < 3; ++c) {
+        v->bbox_min[c] = o->bmin[c];
+        v->bbox_max[c] = o->bmax[c];
+    }
+    v->unitize_scale = o->scale;
+    return NOOR_OK;
+}
+
+void noor_obj_free(noor_obj* o) { delete o; }
+
+// ---- images -----------------------------------------------------------------------------
+
+namespace {
+
+bool read_all(const char* path, std::vector<unsigned char>& buf) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    const long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (n < 0) {
+        std::fclose(f);
+        return false;
+    }
+    buf.resize((size_t)n);
+    const size_t got = n ? std::fread(buf.data(), 1, (size_t)n, f) : 0;
+    std::fclose(f);
+    return got == (size_t)n;
+}
+
+// RGBE texel -> float (Ward's Real Pixels: mantissa + 0.5 times 2^(e - 136))
+void rgbe_to_float(const unsigned char* p, float* o) {
+    if (p[3] == 0) {
+        o[0] = o[1] = o[2] = 0.0f;
+    } else {
+        const float f = std::ldexp(1.0f, (int)p[3] - (128 + 8));
+        o[0] = ((float)p[0] + 0.5f) * f;
+        o[1] = ((float)p[1] + 0.5f) * f;
+        o[2] = ((float)p[2] + 0.5f) * f;
+    }
+    o[3] = 1.0f;
+}
+
+noor_status load_hdr(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
+    size_t p = 0;
+    auto getline_ = [&](std::string& s) {
+        s.clear();
+        while (p < b.size() && b[p] != '\n') s += (char)b[p++];
+        if (p < b.size()) ++p;
+        return !s.empty() || p < b.size();
+    };
+    std::string s;
+    bool fmt = false;
+    while (getline_(s) && !s.empty())
+        if (s.rfind("FORMAT=32-bit_rle_rgbe", 0) == 0) fmt = true;
+    if (!fmt) return fail(NOOR_ERR_UNSUPPORTED, "HDR: only FORMAT=32-bit_rle_rgbe");
+    if (!getline_(s)) return fail(NOOR_ERR_INVALID_PAYLOAD, "HDR: no resolution line");
+    char ya[4] = {0}, xa[4] = {0};
+    int h = 0, w = 0;
+    if (std::sscanf(s.c_str(), "%3s %d %3s %d", ya, &h, xa, &w) != 4 || std::strcmp(ya, "-Y") != 0 ||
+        std::strcmp(xa, "+X") != 0 || w <= 0 || h <= 0 || w > 1 << 16 || h > 1 << 16)
+        return fail(NOOR_ERR_UNSUPPORTED, "HDR: only -Y H +X W orientation");
+    W = (uint32_t)w;
+    H = (uint32_t)h;
+    out.assign(4 * (size_t)W * H, 0.0f);
+    std::vector<unsigned char> line(4 * (size_t)W);
+    for (uint32_t y = 0; y < H; ++y) {
+        if (p + 4 > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "HDR: truncated");
+        const bool rle = W >= 8 && W < 32768 && b[p] == 2 && b[p + 1] == 2 && !(b[p + 2] & 0x80) &&
+                         (((uint32_t)b[p + 2] << 8) | b[p + 3]) == W;
+        if (rle) {
+            p += 4;
+            for (int c = 0; c < 4; ++c) {
+                uint32_t x = 0;
+                while (x < W) {
+                    if (p >= b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "HDR: truncated run");
+                    uint32_t n = b[p++];
+                    if (n > 128) {
+                        n -= 128;
+                        if (n > W - x || p >= b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "HDR: bad run");
+                        const unsigned char v = b[p++];
+                        for (uint32_t k = 0; k < n; ++k) line[4 * (x++) + c] = v;
+                    } else {
+                        if (n == 0 || n > W - x || p + n > b.size())
+                            return fail(NOOR_ERR_INVALID_PAYLOAD, "HDR: bad literal");
+                        for (uint32_t k = 0; k < n; ++k) line[4 * (x++) + c] = b[p++];
+                    }
+                }
+            }
+        } else {
+            if (p + 4 * (size_t)W > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "HDR: truncated");
+            std::memcpy(line.data(), &b[p], 4 * (size_t)W);
+            p += 4 * (size_t)W;
+        }
+        for (uint32_t x = 0; x < W; ++x) rgbe_to_float(&line[4 * x], &out[4 * ((size_t)y * W + x)]);
+    }
+    return NOOR_OK;
+}
+
+bool pnm_token(const std::vector<unsigned char>& b, size_t& p, long& v) {
+    while (p < b.size()) {
+        if (b[p] == '#') {
+            while (p < b.size() && b[p] != '\n') ++p;
+        } else if (std::isspace(b[p])) {
+            ++p;
+        } else {
+            break;
+        }
+    }
+    if (p >= b.size() || !std::isdigit(b[p])) return false;
+    v = 0;
+    while (p < b.size() && std::isdigit(b[p])) {
+        v = v * 10 + (b[p++] - '0');
+        if (v > 1 << 20) return false;
+    }
+    return true;
+}
+
+noor_status load_pnm(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
+    const int ch = b[1] == '6' ? 3 : 1;
+    size_t p = 2;
+    long w, h, mx;
+    if (!pnm_token(b, p, w) || !pnm_token(b, p, h) || !pnm_token(b, p, mx) || w <= 0 || h <= 0 || mx <= 0 ||
+        mx > 65535 || w > 1 << 16 || h > 1 << 16)
+        return fail(NOOR_ERR_INVALID_PAYLOAD, "PNM: bad header");
+    ++p;  // the single whitespace byte after maxval
+    const int bytes = mx > 255 ? 2 : 1;
+    if (p + (size_t)w * h * ch * bytes > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNM: truncated");
+    W = (uint32_t)w;
+    H = (uint32_t)h;
+    out.assign(4 * (size_t)W * H, 1.0f);
+    for (size_t i = 0; i < (size_t)W * H; ++i)
+        for (int c = 0; c < ch; ++c) {
+            long v = b[p++];
+            if (bytes == 2) v = (v << 8) | b[p++];
+            const float f = std::pow((float)v / (float)mx, 2.2f);
+            if (ch == 1) out[4 * i] = out[4 * i + 1] = out[4 * i + 2] = f;
+            else out[4 * i + c] = f;
+        }
+    return NOOR_OK;
+}
+
+noor_status load_pfm(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
+    const int ch = b[1] == 'F' ? 3 : 1;
+    size_t p = 2;
+    long w, h;
+    if (!pnm_token(b, p, w) || !pnm_token(b, p, h) || w <= 0 || h <= 0 || w > 1 << 16 || h > 1 << 16)
+        return fail(NOOR_ERR_INVALID_PAYLOAD, "PFM: bad header");
+    while (p < b.size() && std::isspace(b[p])) ++p;
+    std::string sc;
+    while (p < b.size() && !std::isspace(b[p])) sc += (char)b[p++];
+    ++p;
+    const double scale = std::atof(sc.c_str());
+    if (scale == 0.0) return fail(NOOR_ERR_INVALID_PAYLOAD, "PFM: bad scale");
+    const bool little = scale < 0.0;
+    if (p + (size_t)w * h * ch * 4 > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "PFM: truncated");
+    W = (uint32_t)w;
+    H = (uint32_t)h;
+    out.assign(4 * (size_t)W * H, 1.0f);
+    for (uint32_t r = 0; r < H; ++r) {
+        const uint32_t y = H - 1 - r;  // stored bottom row first
+        for (uint32_t x = 0; x < W; ++x)
+            for (int c = 0; c < ch; ++c) {
+                unsigned char q[4];
+                std::memcpy(q, &b[p], 4);
+                p += 4;
+                if (!little) std::swap(q[0], q[3]), std::swap(q[1], q[2]);
+                float f;
+                std::memcpy(&f, q, 4);
+                float* o = &out[4 * ((size_t)y * W + x)];
+                if (ch == 1) o[0] = o[1] = o[2] = f;
+                else o[c] = f;
+            }
+    }
+    return NOOR_OK;
+}
+
+}  // namespace
+
+noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba) {
+    if (!path || !width || !height || !rgba) return NOOR_ERR_INVALID_ARG;
+    *rgba = nullptr;
+    std::vector<unsigned char> b;
+    if (!read_all(path, b)) return fail(NOOR_ERR_INVALID_ARG, std::string("cannot read ") + path);
+    std::vector<float> px;
+    uint32_t W = 0, H = 0;
+    noor_status s;
+    if (b.size() >= 2 && b[0] == '#' && b[1] == '?') s = load_hdr(b, W, H, px);
+    else if (b.size() >= 2 && b[0] == 'P' && (b[1] == '5' || b[1] == '6')) s = load_pnm(b, W, H, px);
+    else if (b.size() >= 2 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) s = load_pfm(b, W, H, px);
+    else s = fail(NOOR_ERR_UNSUPPORTED, std::string("unsupported image format: ") + path);
+    if (s != NOOR_OK) return s;
+    float* o = new float[px.size()];
+    std::memcpy(o, px.data(), px.size() * sizeof(float));
+    *width = W;
+    *height = H;
+    *rgba = o;
+    return NOOR_OK;
+}
+
+void noor_image_free(float* rgba) { delete[] rgba; }
+
+}  // extern "C"

@@ -24,12 +24,15 @@ STRUCTS = {
     "noor_mesh_desc": abi.MeshDesc, "noor_instance_desc": abi.InstanceDesc, "noor_bvh_params": abi.BvhParams,
     "noor_scene_desc": abi.SceneDesc, "noor_build_stats": abi.BuildStats,
     "noor_view": abi.View, "noor_tune_entry": abi.TuneEntry,
+    "noor_obj_material": abi.ObjMaterial, "noor_obj_view": abi.ObjView,
 }
 
 
 def header_functions():
     names = set()
-    for h in ("noor_hip.h", "noor_scene.h"):
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(noor_\w+)\s*\(", text, flags=re.M):

@@ -1,0 +1,288 @@
+"""Payload producers for files (include/noor_loaders.h), on the CPU.
+
+OBJ + MTL (noor/source/objLoader.cpp:28-147, noor/header/objLoader.h:90-169;
+glm_obj itself is not shipped with the reference, its published behaviour is
+pinned here by closed forms): fan triangulation and negative indices, glmUnitize
+(centre and 2 / max(|max| + |min|) per axis), facet and crease-angle vertex
+normals (a cube keeps its face normals, a sphere gets radial ones), MTL fields
+and map options, light groups as quad lights.  Images
+(noor/source/texture.cpp:49-76): Radiance RGBE written by an independent
+encoder here (flat and run-length scanlines), PFM in both byte orders, PNM
+8/16-bit with stb_image's 2.2 gamma.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from noor_amd import abi, scenes
+from noor_amd.loaders import ObjModel, load_image, obj_scene
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+CUBE = """# unit test cube 2 x 4 x 2 at the origin corner
+mtllib cube.mtl
+v 0 0 0
+v 2 0 0
+v 2 4 0
+v 0 4 0
+v 0 0 2
+v 2 0 2
+v 2 4 2
+v 0 4 2
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+g box
+usemtl red
+f 1/1 4/4 3/3 2/2
+f 5/1 6/2 7/3 8/4
+f 1/1 2/2 6/3 5/4
+usemtl shiny
+f 4 8 7 3
+f 1 5 8 4
+f -7 -6 -2 -1
+g ceiling_light
+usemtl lamp
+f 3 4 8 7
+"""
+
+MTL = """newmtl red
+Kd 0.8 0.1 0.1
+Ks 0 0 0
+Ns 10
+illum 2
+map_Kd -s 2 3 tex.pfm
+newmtl shiny
+Kd 0.1 0.1 0.1
+Ks 0.9 0.9 0.9
+Ni 1.7
+Ns 50
+illum 6
+map_bump -bm 0.25 bump.pgm
+map_d alpha.pgm
+newmtl lamp
+Kd 0 0 0
+Ke 5 4 3
+illum 5
+"""
+
+
+def _pfm(path, img, little=True):
+    h, w, c = img.shape
+    hdr = (b"PF\n" if c == 3 else b"Pf\n") + f"{w} {h}\n{-1.0 if little else 1.0}\n".encode()
+    data = img[::-1].astype("<f4" if little else ">f4").tobytes()
+    with open(path, "wb") as f:
+        f.write(hdr + data)
+
+
+def _pnm(path, img, maxval=255):
+    h, w = img.shape[:2]
+    c = 1 if img.ndim == 2 else 3
+    dt = ">u2" if maxval > 255 else "u1"
+    with open(path, "wb") as f:
+        f.write(f"P{5 if c == 1 else 6}\n# comment\n{w} {h}\n{maxval}\n".encode() + img.astype(dt).tobytes())
+
+
+@pytest.fixture
+def cube_dir(tmp_path):
+    _write(tmp_path / "cube.obj", CUBE)
+    _write(tmp_path / "cube.mtl", MTL)
+    _pfm(tmp_path / "tex.pfm", np.random.default_rng(1).random((4, 8, 3)).astype(np.float32))
+    _pnm(tmp_path / "bump.pgm", (np.arange(16).reshape(4, 4) * 16).astype(np.uint8))
+    _pnm(tmp_path / "alpha.pgm", np.full((2, 2), 255, np.uint8))
+    return tmp_path
+
+
+def test_obj_geometry_unitize_and_fans(cube_dir):
+    m = ObjModel(str(cube_dir / "cube.obj"))
+    assert m.indices.shape == (14, 3)                      # 7 quads as fans
+    np.testing.assert_allclose(m.bbox_min, [0, 0, 0])
+    np.testing.assert_allclose(m.bbox_max, [2, 4, 2])
+    assert m.unitize_scale == pytest.approx(0.5)           # 2 / max(2, 4, 2)
+    p = m.positions
+    np.testing.assert_allclose(p.min(0), [-0.5, -1.0, -0.5], atol=1e-7)
+    np.testing.assert_allclose(p.max(0), [0.5, 1.0, 0.5], atol=1e-7)
+    # first quad 1 4 3 2 -> triangles (1, 4, 3), (1, 3, 2) in file positions
+    src = (np.array([[0, 0, 0], [2, 0, 0], [2, 4, 0], [0, 4, 0], [0, 0, 2], [2, 0, 2], [2, 4, 2], [0, 4, 2]],
+                    np.float64) - [1, 2, 1]) * 0.5
+    tri = p[m.indices]
+    np.testing.assert_allclose(tri[0], src[[0, 3, 2]], atol=1e-7)
+    np.testing.assert_allclose(tri[1], src[[0, 2, 1]], atol=1e-7)
+    # negative indices: f -7 -6 -2 -1 = f 2 3 7 8 ... of 8 vertices: 2 3 7 8
+    np.testing.assert_allclose(tri[10], src[[1, 2, 6]], atol=1e-7)
+    np.testing.assert_allclose(tri[11], src[[1, 6, 7]], atol=1e-7)
+    # texture coordinates: given for the first three quads, (0, 0) elsewhere
+    np.testing.assert_allclose(m.uvs[m.indices[0]], [[0, 0], [0, 1], [1, 1]])
+    np.testing.assert_allclose(m.uvs[m.indices[6]], 0)
+
+
+def test_obj_crease_normals_cube_and_sphere(tmp_path):
+    # cube faces meet at 90 degrees: every corner keeps its facet normal
+    _write(tmp_path / "cube.obj", CUBE.replace("mtllib cube.mtl\n", ""))
+    m = ObjModel(str(tmp_path / "cube.obj"))
+    tri = m.positions[m.indices].astype(np.float64)
+    fn = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    for k in range(3):
+        np.testing.assert_allclose(m.normals[m.indices[:, k]], fn, atol=1e-6)
+    # a subdivided icosahedron (neighbouring faces within 30 degrees): radial normals
+    v, f = scenes.icosphere(2)
+    lines = [f"v {a} {b} {c}" for a, b, c in v] + [f"f {a + 1} {b + 1} {c + 1}" for a, b, c in f]
+    _write(tmp_path / "ico.obj", "\n".join(lines) + "\n")
+    m = ObjModel(str(tmp_path / "ico.obj"), unitize=False)
+    radial = m.positions / np.linalg.norm(m.positions, axis=1, keepdims=True)
+    assert np.min(np.sum(radial * m.normals, axis=1)) > 0.99
+    assert m.positions.shape[0] == v.shape[0]               # one normal per vertex: no splits
+    # at a 5 degree crease no face is within another's: facet normals again
+    m5 = ObjModel(str(tmp_path / "ico.obj"), unitize=False, crease_deg=5.0)
+    tri = m5.positions[m5.indices].astype(np.float64)
+    fn = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    np.testing.assert_allclose(m5.normals[m5.indices[:, 0]], fn, atol=1e-6)
+
+
+def test_mtl_fields_and_lights(cube_dir):
+    m = ObjModel(str(cube_dir / "cube.obj"))
+    mats = {d["name"]: d for d in m.materials}
+    red, shiny, lamp = mats["red"], mats["shiny"], mats["lamp"]
+    assert red["type"] == abi.MAT_DIFFUSE and shiny["type"] == abi.MAT_METAL and lamp["type"] == abi.MAT_EMITTER
+    np.testing.assert_allclose(red["diffuse"], [0.8, 0.1, 0.1], rtol=1e-6)
+    assert red["roughness"] == pytest.approx(0.1) and shiny["roughness"] == pytest.approx(0.02)
+    assert shiny["ior"] == pytest.approx(1.7)
+    assert red["map_kd"] == "tex.pfm" and red["uvscale"] == pytest.approx((2.0, 3.0))
+    assert shiny["map_bump"] == "bump.pgm" and shiny["bumpfactor"] == pytest.approx(0.25)
+    assert shiny["map_d"] == "alpha.pgm"
+    np.testing.assert_allclose(lamp["emittance"], [5, 4, 3])
+    names = [d["name"] for d in m.materials]
+    assert [names[i] for i in m.material] == ["red"] * 6 + ["shiny"] * 6 + ["lamp"] * 2
+    # light group: quad from its first triangle (3 4 8): base = vertex 4, u = v8 - v4, v = v3 - v4
+    assert m.lights.shape == (1, 15)
+    s = 0.5
+    v3, v4, v8 = [(np.array(a, np.float64) - [1, 2, 1]) * s for a in ([2, 4, 0], [0, 4, 0], [0, 4, 2])]
+    L = m.lights[0]
+    np.testing.assert_allclose(L[0:3], v4, atol=1e-7)
+    np.testing.assert_allclose(L[3:6], v8 - v4, atol=1e-7)
+    np.testing.assert_allclose(L[6:9], v3 - v4, atol=1e-7)
+    n = np.cross(v8 - v4, v3 - v4)
+    np.testing.assert_allclose(L[9:12], n / np.linalg.norm(n), atol=1e-7)
+    np.testing.assert_allclose(L[12:15], [5, 4, 3])
+
+
+def _rgbe_encode(img):
+    """Ward's RGBE: shared exponent e = frexp(max) so that max = m 2^e, m in
+    [0.5, 1); bytes = floor(c * 256 / 2^e), exponent byte e + 128."""
+    mx = img[..., :3].max(-1)
+    m, e = np.frexp(mx)
+    out = np.zeros(img.shape[:2] + (4,), np.uint8)
+    ok = mx > 1e-32
+    scale = np.where(ok, 256.0 / np.ldexp(1.0, e), 0)
+    out[..., :3] = np.floor(img[..., :3] * scale[..., None]).clip(0, 255)
+    out[..., 3] = np.where(ok, e + 128, 0)
+    return out
+
+
+def _rgbe_decode(b):
+    f = np.ldexp(1.0, b[..., 3].astype(np.int32) - 136)
+    rgb = (b[..., :3] + 0.5) * f[..., None]
+    return np.where(b[..., 3:4] == 0, 0.0, rgb)
+
+
+def _rle_scanline(row):
+    """Run-length scanline: 2 2 hi lo, then each channel as runs (count > 128:
+    repeat) or literals."""
+    w = row.shape[0]
+    out = bytearray([2, 2, w >> 8, w & 255])
+    for c in range(4):
+        ch = row[:, c]
+        i = 0
+        while i < w:
+            j = i
+            while j < w and j - i < 127 and ch[j] == ch[i]:
+                j += 1
+            if j - i >= 3:
+                out += bytes([128 + (j - i), ch[i]])
+                i = j
+            else:
+                k = i
+                while k < w and k - i < 128 and not (k + 2 < w and ch[k] == ch[k + 1] == ch[k + 2]):
+                    k += 1
+                out += bytes([k - i]) + bytes(ch[i:k])
+                i = k
+    return bytes(out)
+
+
+@pytest.mark.parametrize("rle", [False, True])
+def test_radiance_hdr(tmp_path, rle):
+    rng = np.random.default_rng(3)
+    h, w = 5, 40
+    img = (rng.random((h, w, 3)) * 10 ** rng.uniform(-3, 3, (h, w, 1))).astype(np.float32)
+    img[1, 3:20] = [0.5, 0.25, 4.0]   # runs
+    img[2, 0] = 0                     # zero exponent
+    b = _rgbe_encode(img)
+    body = b"".join(_rle_scanline(b[y]) for y in range(h)) if rle else b.tobytes()
+    p = tmp_path / "x.hdr"
+    with open(p, "wb") as f:
+        f.write(b"#?RADIANCE\n# test\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode() + body)
+    got = load_image(str(p))
+    assert got.shape == (h, w, 4)
+    np.testing.assert_allclose(got[..., :3], _rgbe_decode(b), rtol=1e-6)
+    # RGBE keeps 8 bits relative to the brightest channel of a texel
+    err = np.abs(got[..., :3] - img).max(-1)
+    assert np.all(err <= img.max(-1) / 128 + 1e-30)
+    assert np.all(got[..., 3] == 1)
+
+
+@pytest.mark.parametrize("little", [True, False])
+def test_pfm_and_pnm(tmp_path, little):
+    rng = np.random.default_rng(4)
+    img = rng.standard_normal((3, 5, 3)).astype(np.float32)
+    _pfm(tmp_path / "c.pfm", img, little)
+    np.testing.assert_array_equal(load_image(str(tmp_path / "c.pfm"))[..., :3], img)
+    grey = rng.random((4, 2, 1)).astype(np.float32)
+    _pfm(tmp_path / "g.pfm", grey, little)
+    np.testing.assert_array_equal(load_image(str(tmp_path / "g.pfm"))[..., :3], np.repeat(grey, 3, -1))
+    px = rng.integers(0, 256, (3, 4, 3)).astype(np.uint8)
+    _pnm(tmp_path / "c.ppm", px)
+    np.testing.assert_allclose(load_image(str(tmp_path / "c.ppm"))[..., :3], (px / 255.0) ** 2.2, rtol=1e-5, atol=1e-7)
+    g16 = rng.integers(0, 4096, (2, 3)).astype(np.uint16)
+    _pnm(tmp_path / "g.pgm", g16, maxval=4095)
+    np.testing.assert_allclose(load_image(str(tmp_path / "g.pgm"))[..., 0], (g16 / 4095.0) ** 2.2, rtol=1e-5,
+                               atol=1e-7)
+
+
+def test_bad_files_fail_cleanly(tmp_path):
+    _write(tmp_path / "bad.obj", "v 0 0 0\nf 1 2 3\n")
+    with pytest.raises(abi.NoorError):
+        ObjModel(str(tmp_path / "bad.obj"))
+    with pytest.raises(abi.NoorError):
+        ObjModel(str(tmp_path / "missing.obj"))
+    with open(tmp_path / "t.hdr", "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 4 +X 4\n\x01\x02")
+    with pytest.raises(abi.NoorError):
+        load_image(str(tmp_path / "t.hdr"))
+    with open(tmp_path / "x.png", "wb") as f:
+        f.write(b"\x89PNG\r\n")
+    with pytest.raises(abi.NoorError) as e:
+        load_image(str(tmp_path / "x.png"))
+    assert e.value.status == abi.ERR_UNSUPPORTED
+
+
+def test_obj_scene_renders_with_the_oracle(cube_dir):
+    from noor_amd.scene import make_camera, make_options, make_spec
+    from noor_amd.scenes import Workload
+    from oracle.binding import OracleScene
+    sb, m = obj_scene(str(cube_dir / "cube.obj"))
+    payload = sb.build()
+    assert payload.view.num_triangles == 14 and payload.view.num_area_lights == 1
+    cam = make_camera((2.0, 1.5, 3.0), (0.0, 0.0, 0.0), (0, 1, 0), np.radians(50.0), 24, 20)
+    spec = make_spec(payload, bounces=3, rr=3, skydome_type=abi.SKY_HDR)
+    wl = Workload("obj", payload, cam, spec, make_options(), None, 1, info={})
+    f, _ = OracleScene.from_workload(wl).render(1, 2)
+    assert np.isfinite(f).all() and f[..., :3].max() > 0
