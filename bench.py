@@ -41,7 +41,7 @@ WIDE_NODE_BYTES = 56  # per wide-node visit: origin + steps (16 B), 8-bit child 
 # HBM bytes per dispatch of the dominant kernel on the default terrain workload,
 # from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB
 # calibration copy (commands and figures in profiles/pmc_traffic_terrain_1080p.md)
-MEASURED_HBM_BYTES = {("terrain", 1920, 1080, 32, "wave_extend"): 1167801571}
+MEASURED_HBM_BYTES = {("terrain", 1920, 1080, 32, "wave_extend"): 1167666429}
 
 
 def log(msg):
@@ -274,11 +274,15 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-textured", action="store_true")
+    ap.add_argument("--textured-only", action="store_true", help="run only the texture-pipeline leg (profiling)")
     ap.add_argument("--no-build-compare", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe()
+        return
+    if args.textured_only:
+        print(json.dumps({"textured": bench_textured()}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
