@@ -186,7 +186,8 @@ typedef struct noor_distant_light { noor_float3 ke, power; int32_t type; noor_fl
  * bilinear within a level and linear between levels
  * (noor/cuda/texture.cuh:30-400, noor/cuda/image.cuh:28-125).
  * The last texture of the payload is the environment / sky texture (any
- * size, clamp addressing, bilinear at level 0). */
+ * size up to 65536 a side, resampled the same way; clamp addressing,
+ * bilinear at level 0). */
 typedef struct noor_texture_view {
     uint32_t width, height;
     const float* rgba;      /* width*height*4 floats */

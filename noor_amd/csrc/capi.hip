@@ -163,6 +163,8 @@ noor_status validate(Context* c, const noor_payload_view* P) {
     }
     const noor_texture_view& env = P->textures[P->num_textures - 1];
     if (!env.rgba || env.width == 0 || env.height == 0) return fail(c, NOOR_ERR_INVALID_PAYLOAD, "no environment texture");
+    if (env.width > 1u << 16 || env.height > 1u << 16)
+        return fail(c, NOOR_ERR_UNSUPPORTED, "environment textures are at most 65536 texels a side");
     if (P->num_infinite_lights > 1) return fail(c, NOOR_ERR_UNSUPPORTED, "more than one infinite light");
     return NOOR_OK;
 }
@@ -483,12 +485,21 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
     S.num_lights = (int)(P->num_area_lights + P->num_point_lights + P->num_spot_lights + P->num_distant_lights +
                          P->num_infinite_lights);
     {
+        // the environment is a CudaMipMap too (noor/cuda/skydome.cuh:40-52): sides
+        // resampled to the nearest power of two like the material textures
         const noor_texture_view& env = P->textures[P->num_textures - 1];
-        if ((st = dev_upload(c, (noor_float4**)&S.env, (const noor_float4*)env.rgba, (size_t)env.width * env.height)) !=
-            NOOR_OK)
+        const uint32_t ew = noor::nearest_pow2(env.width), eh = noor::nearest_pow2(env.height);
+        std::vector<float> resized;
+        const float* src = env.rgba;
+        if (ew != env.width || eh != env.height) {
+            resized = noor::lanczos_resize(env.rgba, env.width, env.height, ew, eh,
+                                           ((options ? options->compat : NOOR_COMPAT_ALL) & NOOR_COMPAT_RESIZE_SUM) != 0);
+            src = resized.data();
+        }
+        if ((st = dev_upload(c, (noor_float4**)&S.env, (const noor_float4*)src, (size_t)ew * eh)) != NOOR_OK)
             return bail(st);
-        S.env_w = env.width;
-        S.env_h = env.height;
+        S.env_w = ew;
+        S.env_h = eh;
     }
     noor_hosek hk{};
     if (hosek) hk = *hosek;

@@ -523,7 +523,7 @@ def quad_uv(corner, u, v):
 
 
 def textured(seed=8, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, lens_radius=0.0,
-             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL, tex=64, flat=False):
+             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL, tex=64, flat=False, env_size=(128, 64)):
     """The image-texture pipeline of the reference: mip-mapped colour,
     roughness and emittance textures read at the pixel footprint (ray
     differentials from the camera, carried through mirror, metal and glass
@@ -582,7 +582,7 @@ def textured(seed=8, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, len
     tp, tt, tn, tu = torus(0.22, 0.08, 48, 24)
     R = random_rotation(rng)
     sb.add_mesh(Mesh(tp @ R.T + np.array([-0.1, -0.3, -0.75]), tt, normals=tn @ R.T, uvs=tu, material=mats["torus"]))
-    sb.set_env(env_map(128, 64, seed + 3, lobes=2, sigma_deg=8.0, peak=20.0))
+    sb.set_env(env_map(env_size[0], env_size[1], seed + 3, lobes=2, sigma_deg=8.0, peak=20.0))
     payload = sb.build()
     cam = make_camera((0.0, 0.0, 0.95), (0.0, -0.15, -1.0), (0, 1, 0), np.radians(60.0), width, height,
                       lens_radius=lens_radius)

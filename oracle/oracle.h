@@ -75,6 +75,7 @@ typedef struct orc_scene {
     /* environment texture (the last payload texture) */
     uint32_t env_w, env_h;
     const float* env;            /* RGBA32F */
+    float* env_resized;          /* owned copy when the payload's sides were not powers of two */
     float* sky_bake;             /* PHYSICAL: sin(theta) * sky baked texture */
     uint32_t sky_w, sky_h;
     orc_dist2d dist;
@@ -101,7 +102,8 @@ void oracle_scene_destroy(orc_scene* s);
 void oracle_scene_set_camera(orc_scene* s, const noor_camera* cam);
 void oracle_scene_set_spec(orc_scene* s, const noor_spec* spec);
 void oracle_scene_set_options(orc_scene* s, const noor_options* opt);
-/* Replace the environment texture (same size) and rebuild its distribution. */
+/* Replace the environment texture (at its stored size, after resampling to
+ * powers of two) and rebuild its distribution. */
 int oracle_scene_set_env(orc_scene* s, const float* rgba);
 /* Replace per-material diffuse rgb and roughness xy (adjoint tests). */
 void oracle_scene_set_material_params(orc_scene* s, int mat, const float* diffuse,

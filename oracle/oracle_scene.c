@@ -556,10 +556,18 @@ int oracle_scene_create(const noor_payload_view* p, const noor_hosek* hosek,
         free(store_idx);
     }
     if (p->num_textures > 0) {
+        /* the environment is a CudaMipMap too (noor/cuda/skydome.cuh:40-52):
+         * sides resampled to the nearest power of two */
         const noor_texture_view* t = &p->textures[p->num_textures - 1];
-        s->env_w = t->width;
-        s->env_h = t->height;
+        const uint32_t W = nearest_pow2u(t->width), H = nearest_pow2u(t->height);
+        s->env_w = W;
+        s->env_h = H;
         s->env = t->rgba;
+        if (t->rgba && (W != t->width || H != t->height)) {
+            s->env_resized = lanczos_resize(t->rgba, t->width, t->height, W, H,
+                                            (opt->compat & NOOR_COMPAT_RESIZE_SUM) != 0);
+            s->env = s->env_resized;
+        }
     }
     s->num_lights = (int)(p->num_area_lights + p->num_point_lights + p->num_spot_lights +
                           p->num_distant_lights + p->num_infinite_lights);
@@ -577,6 +585,7 @@ void oracle_scene_destroy(orc_scene* s) {
     free(s->mats);
     free(s->tex_texels);
     free(s->tex_desc);
+    free(s->env_resized);
     free(s);
 }
 

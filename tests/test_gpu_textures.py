@@ -39,10 +39,11 @@ def test_textured_megakernel_and_wavefront_vs_oracle(lens, mis, tri):
 
 @pytest.mark.parametrize("compat", [abi.COMPAT_ALL, 0])
 def test_resized_textures_vs_oracle(compat):
-    """48x48 and 24x24 images, resampled to 32x32 and 16x16 at creation (the
-    reference's summed Lanczos passes under compat, composed passes without):
-    the device store and the oracle's agree, so the films do."""
-    wl = scenes.textured(tex=48, compat=compat)
+    """48x48 and 24x24 images and a 96x40 environment, resampled to 32x32,
+    16x16 and 64x32 at creation (the reference's summed Lanczos passes under
+    compat, composed passes without): the device's copies and the oracle's
+    agree, so the films do."""
+    wl = scenes.textured(tex=48, compat=compat, env_size=(96, 40))
     r = Renderer.from_workload(wl)
     r.render(spp=2)
     g = r.film()

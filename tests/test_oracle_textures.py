@@ -336,6 +336,22 @@ def test_lanczos_resize_of_non_power_of_two_textures(summed):
             np.testing.assert_allclose(got[Y, X, 0], expect, rtol=2e-5)
 
 
+@pytest.mark.parametrize("summed", [True, False])
+def test_environment_is_resampled_like_textures(summed):
+    """A 40 x 96 environment becomes 32 x 64 (nearest powers of two, ties to
+    the smaller); a constant sky seen directly doubles under the summed passes."""
+    env = np.full((40, 96, 4), 0.5, np.float32)
+    cam = make_camera((0.0, 1.0, 0.0), (0.0, 5.0, 0.0), (0, 0, -1), np.radians(40.0), 8, 6)
+    wl = _plane_scene(material(abi.MAT_DIFFUSE), cam=cam, env=env)
+    wl.options.compat = abi.COMPAT_ALL if summed else 0
+    wl.spec.lighting_type |= abi.LIGHT_SKY_MASK
+    o = OracleScene.from_workload(wl)
+    w, h, func, cdf = o.dist()
+    assert (w, h) == (64, 32)
+    f, _ = o.render(1, 1)
+    np.testing.assert_allclose(f[..., :3], 1.0 if summed else 0.5, rtol=2e-6)
+
+
 def test_payload_texture_validation():
     """Material textures are at most 32768 texels a side; the check runs
     before any device call."""
