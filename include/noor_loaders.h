@@ -11,8 +11,8 @@
  * The reference parses OBJ through the glm_obj library, which it does not
  * ship; the behaviour restated here is that library's published one
  * (glmReadOBJ, glmUnitize, glmFacetNormals, glmVertexNormals), see
- * noor_obj_load.  FBX (Autodesk SDK), EXR (tinyexr) and the 8-bit formats
- * of stb_image other than PNM are not provided.
+ * noor_obj_load.  FBX (Autodesk SDK), EXR (tinyexr), JPEG and the other
+ * 8-bit formats of stb_image besides PNG and PNM are not provided.
  */
 #ifndef NOOR_LOADERS_H
 #define NOOR_LOADERS_H
@@ -84,8 +84,11 @@ void noor_obj_free(noor_obj* obj);
 /* Reads an image as RGBA32F, row 0 first: Radiance RGBE (.hdr, flat or
  * run-length scanlines, -Y +X orientation), PFM (colour or grey, either byte
  * order, stored bottom row first), binary PNM (P5 grey, P6 colour, 8 or 16
- * bits, converted as stb_image's loadf does: (v / max)^2.2).  Alpha is 1.
- * The returned buffer is freed with noor_image_free. */
+ * bits: (v / max)^2.2), PNG (non-interlaced; every colour type and bit
+ * depth; samples cut to 8 bits, then colour (v / 255)^2.2 and alpha v / 255,
+ * as stb_image's loadf converts them).  Grey is replicated to rgb; alpha is 1
+ * where the file has none.  The returned buffer is freed with
+ * noor_image_free. */
 noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba);
 void noor_image_free(float* rgba);
 

@@ -54,7 +54,7 @@ def build(verbose=True, jobs=8):
             if verbose:
                 print(f"  compiled {os.path.basename(cmd[-1])} in {dt:.1f}s", flush=True)
     tmp = OUT + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread", "-lz"])
     os.replace(tmp, OUT)
     if verbose:
         print(f"  linked {OUT}", flush=True)

@@ -20,6 +20,8 @@
 #include <tuple>
 #include <vector>
 
+#include <zlib.h>
+
 struct noor_obj {
     std::vector<float> pos, nrm, uv;
     std::vector<uint32_t> idx, mat;
@@ -589,6 +591,111 @@ noor_status load_pfm(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     return NOOR_OK;
 }
 
+uint32_t be32(const unsigned char* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+// PNG (ISO/IEC 15948): non-interlaced, every colour type and bit depth.
+// Samples go to 8 bits first (16-bit ones keep their high byte), then to
+// float as stb_image's loadf does: colour (v / 255)^2.2, alpha v / 255.
+noor_status load_png(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
+    size_t p = 8;
+    uint32_t w = 0, h = 0;
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<unsigned char> idat, plte, trns;
+    while (p + 8 <= b.size()) {
+        const uint32_t len = be32(&b[p]);
+        if (len > b.size() - p - 8 || b.size() - p - 8 - len < 4) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: truncated chunk");
+        const std::string type(reinterpret_cast<const char*>(&b[p + 4]), 4);
+        const unsigned char* d = &b[p + 8];
+        if (type == "IHDR") {
+            if (len < 13) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad IHDR");
+            w = be32(d);
+            h = be32(d + 4);
+            depth = d[8];
+            ctype = d[9];
+            interlace = d[12];
+        } else if (type == "PLTE") {
+            plte.assign(d, d + len);
+        } else if (type == "tRNS") {
+            trns.assign(d, d + len);
+        } else if (type == "IDAT") {
+            idat.insert(idat.end(), d, d + len);
+        } else if (type == "IEND") {
+            break;
+        }
+        p += 12 + (size_t)len;
+    }
+    const int chans = ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : ctype == 6 ? 4 : 0;
+    const bool depth_ok = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                          (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
+                          ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+    if (!chans || !depth_ok || w == 0 || h == 0 || w > 1 << 16 || h > 1 << 16)
+        return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad header");
+    if (interlace) return fail(NOOR_ERR_UNSUPPORTED, "PNG: interlaced images are not supported");
+    if (ctype == 3 && (plte.empty() || plte.size() % 3)) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: no palette");
+    const size_t bits = (size_t)chans * depth;
+    const size_t stride = ((size_t)w * bits + 7) / 8, bpp = std::max<size_t>(1, bits / 8);
+    std::vector<unsigned char> raw((stride + 1) * h);
+    uLongf n = (uLongf)raw.size();
+    if (uncompress(raw.data(), &n, idat.data(), (uLong)idat.size()) != Z_OK || n != raw.size())
+        return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad image data");
+    // unfilter in place (filters 0-4: none, sub, up, average, Paeth)
+    std::vector<unsigned char> img(stride * h), prev(stride, 0);
+    for (uint32_t y = 0; y < h; ++y) {
+        const unsigned char f = raw[y * (stride + 1)];
+        const unsigned char* src = &raw[y * (stride + 1) + 1];
+        unsigned char* cur = &img[y * stride];
+        for (size_t i = 0; i < stride; ++i) {
+            const int a = i >= bpp ? cur[i - bpp] : 0, up = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+            int v = src[i];
+            switch (f) {
+                case 0: break;
+                case 1: v += a; break;
+                case 2: v += up; break;
+                case 3: v += (a + up) >> 1; break;
+                case 4: {
+                    const int pp = a + up - c, pa = std::abs(pp - a), pb = std::abs(pp - up), pc = std::abs(pp - c);
+                    v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? up : c);
+                    break;
+                }
+                default: return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad filter");
+            }
+            cur[i] = (unsigned char)v;
+        }
+        std::memcpy(prev.data(), cur, stride);
+    }
+    W = w;
+    H = h;
+    out.assign(4 * (size_t)w * h, 1.0f);
+    auto sample = [&](const unsigned char* row, uint32_t x, int c) -> int {  // 8-bit value of sample c of texel x
+        if (depth == 16) return row[2 * ((size_t)x * chans + c)];
+        if (depth == 8) return row[(size_t)x * chans + c];
+        const size_t bit = (size_t)x * depth;
+        const int v = (row[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+        return ctype == 3 ? v : v * (255 / ((1 << depth) - 1));
+    };
+    const float g = 2.2f;
+    for (uint32_t y = 0; y < h; ++y) {
+        const unsigned char* row = &img[y * stride];
+        for (uint32_t x = 0; x < w; ++x) {
+            float* o = &out[4 * ((size_t)y * w + x)];
+            if (ctype == 3) {
+                const int i = sample(row, x, 0);
+                if ((size_t)i * 3 + 2 >= plte.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: palette index");
+                for (int c = 0; c < 3; ++c) o[c] = std::pow(plte[3 * i + c] / 255.0f, g);
+                o[3] = (size_t)i < trns.size() ? trns[i] / 255.0f : 1.0f;
+            } else if (chans <= 2) {
+                const float v = std::pow(sample(row, x, 0) / 255.0f, g);
+                o[0] = o[1] = o[2] = v;
+                if (chans == 2) o[3] = sample(row, x, 1) / 255.0f;
+            } else {
+                for (int c = 0; c < 3; ++c) o[c] = std::pow(sample(row, x, c) / 255.0f, g);
+                if (chans == 4) o[3] = sample(row, x, 3) / 255.0f;
+            }
+        }
+    }
+    return NOOR_OK;
+}
+
 }  // namespace
 
 noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba) {
@@ -602,6 +709,7 @@ noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height,
     if (b.size() >= 2 && b[0] == '#' && b[1] == '?') s = load_hdr(b, W, H, px);
     else if (b.size() >= 2 && b[0] == 'P' && (b[1] == '5' || b[1] == '6')) s = load_pnm(b, W, H, px);
     else if (b.size() >= 2 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) s = load_pfm(b, W, H, px);
+    else if (b.size() >= 8 && std::memcmp(b.data(), "\x89PNG\r\n\x1a\n", 8) == 0) s = load_png(b, W, H, px);
     else s = fail(NOOR_ERR_UNSUPPORTED, std::string("unsupported image format: ") + path);
     if (s != NOOR_OK) return s;
     float* o = new float[px.size()];

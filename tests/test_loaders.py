@@ -257,6 +257,86 @@ def test_pfm_and_pnm(tmp_path, little):
                                atol=1e-7)
 
 
+def _png(path, arr, ctype, depth=8, palette=None, trns=None, seed=0):
+    """A PNG written here: rows packed at `depth`, each with a random filter
+    type (the filter arithmetic of ISO/IEC 15948 applied forward)."""
+    import zlib
+    rng = np.random.default_rng(seed)
+    h, w = arr.shape[:2]
+    chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    a = arr.reshape(h, w * chans).astype(np.int64)
+    if depth == 16:
+        rows = [np.stack([r >> 8, r & 255], -1).reshape(-1) for r in a]
+    elif depth == 8:
+        rows = list(a)
+    else:
+        per = 8 // depth
+        rows = []
+        for r in a:
+            pad = np.zeros(-(-len(r) // per) * per, np.int64)
+            pad[:len(r)] = r
+            rows.append(np.array([sum(int(v) << (8 - depth * (k + 1)) for k, v in enumerate(pad[i:i + per]))
+                                  for i in range(0, len(pad), per)]))
+    bpp = max(1, chans * depth // 8)
+    raw, prev = b"", np.zeros(len(rows[0]), np.int64)
+    for r in rows:
+        f = int(rng.integers(0, 5))
+        left = np.concatenate([np.zeros(bpp, np.int64), r[:-bpp]])
+        ul = np.concatenate([np.zeros(bpp, np.int64), prev[:-bpp]])
+        if f == 0:
+            e = r
+        elif f == 1:
+            e = r - left
+        elif f == 2:
+            e = r - prev
+        elif f == 3:
+            e = r - (left + prev) // 2
+        else:
+            p_ = left + prev - ul
+            pa, pb, pc = np.abs(p_ - left), np.abs(p_ - prev), np.abs(p_ - ul)
+            e = r - np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, prev, ul))
+        raw += bytes([f]) + bytes((e % 256).astype(np.uint8))
+        prev = r
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xffffffff)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
+    if palette is not None:
+        png += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
+    if trns is not None:
+        png += chunk(b"tRNS", bytes(np.asarray(trns, np.uint8)))
+    png += chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b"")
+    with open(path, "wb") as fh:
+        fh.write(png)
+
+
+@pytest.mark.parametrize("ctype,depth", [(2, 8), (6, 8), (0, 8), (4, 8), (2, 16), (0, 4), (0, 1), (3, 8), (3, 2)])
+def test_png(tmp_path, ctype, depth):
+    rng = np.random.default_rng(ctype * 100 + depth)
+    h, w = 7, 13
+    chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    top = (1 << depth) - 1
+    if ctype == 3:
+        pal = rng.integers(0, 256, (1 << depth, 3))
+        trns = rng.integers(0, 256, 1 << depth)
+        idx = rng.integers(0, 1 << depth, (h, w))
+        _png(tmp_path / "p.png", idx, ctype, depth, pal, trns)
+        got = load_image(str(tmp_path / "p.png"))
+        np.testing.assert_allclose(got[..., :3], (pal[idx] / 255.0) ** 2.2, rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(got[..., 3], trns[idx] / 255.0, rtol=1e-6)
+        return
+    arr = rng.integers(0, top + 1, (h, w, chans))
+    _png(tmp_path / "p.png", arr, ctype, depth)
+    got = load_image(str(tmp_path / "p.png"))
+    v8 = arr >> 8 if depth == 16 else arr * (255 // top)
+    col = (v8 / 255.0) ** 2.2
+    if chans <= 2:
+        np.testing.assert_allclose(got[..., :3], np.repeat(col[..., :1], 3, -1), rtol=1e-5, atol=1e-7)
+    else:
+        np.testing.assert_allclose(got[..., :3], col[..., :3], rtol=1e-5, atol=1e-7)
+    alpha = v8[..., -1] / 255.0 if chans in (2, 4) else np.ones((h, w))
+    np.testing.assert_allclose(got[..., 3], alpha, rtol=1e-6)
+
+
 def test_bad_files_fail_cleanly(tmp_path):
     _write(tmp_path / "bad.obj", "v 0 0 0\nf 1 2 3\n")
     with pytest.raises(abi.NoorError):
@@ -267,11 +347,15 @@ def test_bad_files_fail_cleanly(tmp_path):
         f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 4 +X 4\n\x01\x02")
     with pytest.raises(abi.NoorError):
         load_image(str(tmp_path / "t.hdr"))
-    with open(tmp_path / "x.png", "wb") as f:
-        f.write(b"\x89PNG\r\n")
+    with open(tmp_path / "x.jpg", "wb") as f:
+        f.write(b"\xff\xd8\xff\xe0")
     with pytest.raises(abi.NoorError) as e:
-        load_image(str(tmp_path / "x.png"))
+        load_image(str(tmp_path / "x.jpg"))
     assert e.value.status == abi.ERR_UNSUPPORTED
+    with open(tmp_path / "x.png", "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n\x00\x00")
+    with pytest.raises(abi.NoorError):
+        load_image(str(tmp_path / "x.png"))
 
 
 def test_obj_scene_renders_with_the_oracle(cube_dir):
