@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-textured", action="store_true")
+    ap.add_argument("--tri", default="watertight", choices=["watertight", "wald"],
+                    help="triangle test (wald: the reference's projection test)")
     ap.add_argument("--textured-only", action="store_true", help="run only the texture-pipeline leg (profiling)")
     ap.add_argument("--no-build-compare", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
@@ -299,6 +301,8 @@ def main():
         kw["spp"] = args.spp
     wl = fn(**kw)
     spp = wl.spp
+    if args.tri == "wald":
+        wl.options.tri_test = 1
     tri_test = wl.options.tri_test
     log(f"scene {args.workload}: {wl.payload.view.num_triangles} triangles, BVH {wl.payload.stats.num_nodes} nodes, "
         f"build {wl.payload.stats.build_seconds:.1f}s, total setup {time.time() - t0:.1f}s")
