@@ -623,7 +623,6 @@ noor_status noor_set_wavefront_params(noor_ctx* ctx, uint32_t queue_capacity, in
 noor_status noor_render_wavefront(noor_ctx* ctx, uint32_t* frame, uint32_t spp) {
     if (!ctx || !frame || *frame < 1 || spp == 0) return NOOR_ERR_INVALID_ARG;
     Context* c = &ctx->c;
-    if (c->S.mis) return fail(c, NOOR_ERR_UNSUPPORTED, "the wavefront renderer does not implement MIS");
     HIP_TRY(c, noor::wave_render(*c, *frame, spp));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->timer.collect();
@@ -652,7 +651,6 @@ noor_status check_views(Context* c, const noor_view* views, uint32_t n) {
         pixels += (uint64_t)v.camera.w * v.camera.h;
     }
     if (pixels >= (1ull << 31)) return fail(c, NOOR_ERR_INVALID_ARG, "batch larger than 2^31 pixels");
-    if (c->S.mis) return fail(c, NOOR_ERR_UNSUPPORTED, "the wavefront renderer does not implement MIS");
     return NOOR_OK;
 }
 

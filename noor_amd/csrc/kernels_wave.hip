@@ -101,6 +101,12 @@ struct WaveBuffers {
     float4* nee_beta = nullptr;
     float4* sh_o = nullptr;    // shadow ray origin, tmax
     float4* sh_d = nullptr;
+    // MIS: the BSDF-sampled candidate and its shadow ray; w of nee_a / nee_b
+    // flags which of the two candidates need their occlusion test
+    float4* nee_b = nullptr;
+    float4* sh2_o = nullptr;
+    float4* sh2_d = nullptr;
+    bool mis = false;
     uint32_t* q_ray[2] = {nullptr, nullptr};
     uint32_t* q_free[2] = {nullptr, nullptr};
     uint32_t* q_class = nullptr;  // kClasses * capacity
@@ -449,12 +455,19 @@ __global__ __launch_bounds__(256) void k_shade(const DevScene* __restrict__ Sp, 
         // the megakernel traces at most `bounces` extension rays per path
         if (cont && ps.bounce + 1 >= (uint32_t)S.bounces) cont = false;
         ended = !cont;
-        if (nee.has && nee.na) {
+        if (nee.has && (nee.na || nee.nb)) {
             shadow = true;
-            B.nee_a[p] = make_float4(nee.a.x, nee.a.y, nee.a.z, 0.0f);
+            B.nee_a[p] = make_float4(nee.a.x, nee.a.y, nee.a.z, __int_as_float(nee.na));
             B.nee_beta[p] = make_float4(nee.beta.x, nee.beta.y, nee.beta.z, 0.0f);
             B.sh_o[p] = make_float4(nee.ra.o.x, nee.ra.o.y, nee.ra.o.z, nee.ra.tmax);
             B.sh_d[p] = make_float4(nee.ra.d.x, nee.ra.d.y, nee.ra.d.z, 0.0f);
+            if (nee.nb) {  // only with MIS
+                B.nee_b[p] = make_float4(nee.b.x, nee.b.y, nee.b.z, 1.0f);
+                B.sh2_o[p] = make_float4(nee.rb.o.x, nee.rb.o.y, nee.rb.o.z, nee.rb.tmax);
+                B.sh2_d[p] = make_float4(nee.rb.d.x, nee.rb.d.y, nee.rb.d.z, 0.0f);
+            } else if (S.mis) {
+                B.nee_b[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
         } else {
             resolve_nee(ps, nee, false, false);
         }
@@ -473,7 +486,9 @@ __global__ __launch_bounds__(256) void k_shade(const DevScene* __restrict__ Sp, 
     if (ended) B.q_free[nxt][s3] = p;
 }
 
-template <bool STATS, bool ALPHA>
+// MIS: a task may hold a second, BSDF-sampled shadow ray (separate
+// instantiation, so the kernel without MIS keeps its registers)
+template <bool STATS, bool ALPHA, bool MIS>
 __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStats* stats) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
@@ -484,29 +499,49 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
     bool overflow = false, have = false, exhausted = false;
     uint32_t shd = 0;  // rays this lane started
     uint32_t p = 0;
+    // a task is one path's NEE: its light-sampled ray (a), then, with MIS,
+    // its BSDF-sampled ray (b), traced one after the other by the same lane
+    bool second = false, occ_a = false;
     AnyTrav t;
     for (;;) {
         const uint32_t idx = refill(&ctl->work_shadow, count, have, exhausted);
         if (idx != 0xFFFFFFFFu) {
             p = B.q_shadow[idx];
-            const float4 o = B.sh_o[p], d = B.sh_d[p];
+            occ_a = false;
+            second = MIS && __float_as_int(B.nee_a[p].w) == 0;  // no light-sampled ray: the BSDF-sampled one only
+            const float4 o = second ? B.sh2_o[p] : B.sh_o[p], d = second ? B.sh2_d[p] : B.sh_d[p];
             any_begin(S, mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, o.w), t);
             have = true;
             ++shd;
         }
         if (__ballot(have) == 0ull) break;
         if (have && any_steps<STATS, ALPHA>(S, t, st, cnt, overflow, B.steps)) {
+            const bool nb = MIS && B.nee_b[p].w != 0.0f;
+            if (!second && nb) {
+                occ_a = t.occluded;
+                second = true;
+                const float4 o = B.sh2_o[p], d = B.sh2_d[p];
+                any_begin(S, mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, o.w), t);
+                ++shd;
+                continue;
+            }
             Nee nee;
             const float4 a = B.nee_a[p], bt = B.nee_beta[p];
             nee.a = f3{a.x, a.y, a.z};
             nee.beta = f3{bt.x, bt.y, bt.z};
             nee.has = 1;
-            nee.na = 1;
-            nee.nb = 0;
+            nee.na = MIS ? __float_as_int(a.w) : 1;
+            nee.nb = nb ? 1 : 0;
+            if (nb) {
+                const float4 b = B.nee_b[p];
+                nee.b = f3{b.x, b.y, b.z};
+            }
+            const bool oa = nee.na ? (second && nb ? occ_a : t.occluded) : false;
+            const bool ob = nb ? t.occluded : false;
             PathState ps;
             const float4 L = B.L[p];
             ps.L = f3{L.x, L.y, L.z};
-            resolve_nee(ps, nee, t.occluded, false);
+            resolve_nee(ps, nee, oa, ob);
             B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
             have = false;
         }
@@ -549,7 +584,7 @@ __global__ void k_finish(WaveBuffers B, float4* films, uint32_t s0, uint32_t s1,
 // host side
 
 static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, uint32_t max_views,
-                             uint32_t max_segments, bool tex) {
+                             uint32_t max_segments, bool tex, bool mis) {
     auto alloc = [&](void** p, size_t bytes) -> hipError_t {
         hipError_t e = hipMalloc(p, bytes);
         if (e == hipSuccess) B.allocs.push_back(*p);
@@ -581,6 +616,12 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
     WA(B.nee_beta, cap);
     WA(B.sh_o, cap);
     WA(B.sh_d, cap);
+    if (mis) {
+        WA(B.nee_b, cap);
+        WA(B.sh2_o, cap);
+        WA(B.sh2_d, cap);
+    }
+    B.mis = mis;
     WA(B.q_ray[0], cap);
     WA(B.q_ray[1], cap);
     WA(B.q_free[0], cap);
@@ -623,8 +664,10 @@ static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
     auto shade = tex ? k_shade<true> : k_shade<false>;
     auto extend = c.opt.stats ? (tex ? k_extend<true, true> : k_extend<true, false>)
                               : (tex ? k_extend<false, true> : k_extend<false, false>);
-    auto shadow = c.opt.stats ? (tex ? k_shadow<true, true> : k_shadow<true, false>)
-                              : (tex ? k_shadow<false, true> : k_shadow<false, false>);
+    auto shadow = c.S.mis ? (c.opt.stats ? (tex ? k_shadow<true, true, true> : k_shadow<true, false, true>)
+                                         : (tex ? k_shadow<false, true, true> : k_shadow<false, false, true>))
+                          : (c.opt.stats ? (tex ? k_shadow<true, true, false> : k_shadow<true, false, false>)
+                                         : (tex ? k_shadow<false, true, false> : k_shadow<false, false, false>));
     for (int i = 0; i < iters; ++i) {
         const int cur = i & 1;
         T.begin(c.stream, "wave_regen");
@@ -688,13 +731,13 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
     const uint32_t seg_need = (uint32_t)std::max<uint64_t>((uint64_t)max_spp * nv, 1);
     if (!c.wave || c.wave->capacity != capacity || c.wave->sb_items < sb_need || c.wave->max_views < nv ||
         c.wave->max_segments < seg_need || c.wave->block != block || c.wave->stack_lds != c.S.stack_lds ||
-        c.wave->tex != (c.S.textured != 0)) {
+        c.wave->tex != (c.S.textured != 0) || c.wave->mis != (c.S.mis != 0)) {
         wave_free(c);
         c.wave = new WaveBuffers();
         c.wave->block = block;
         c.wave->stack_lds = c.S.stack_lds;
         e = wave_alloc(*c.wave, capacity, (uint32_t)sb_need, std::max<uint32_t>(nv, 64), std::max<uint32_t>(seg_need, 256),
-                       c.S.textured != 0);
+                       c.S.textured != 0, c.S.mis != 0);
         if (e != hipSuccess) {
             wave_free(c);
             return e;

@@ -31,10 +31,9 @@ def test_textured_megakernel_and_wavefront_vs_oracle(lens, mis, tri):
           f"identical {np.mean(np.all(g == c, axis=-1)):.4f}", flush=True)
     assert err <= TOL_IMAGE
     assert float(g[..., :3].mean()) > 0.05
-    if not mis:
-        w = Renderer.from_workload(wl)
-        w.render_wavefront(spp=4)
-        np.testing.assert_array_equal(w.film(), g)
+    w = Renderer.from_workload(wl)
+    w.render_wavefront(spp=4)
+    np.testing.assert_array_equal(w.film(), g)
 
 
 @pytest.mark.parametrize("compat", [abi.COMPAT_ALL, 0])
