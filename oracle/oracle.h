@@ -19,6 +19,13 @@
  *   - Environment texture lookups are software bilinear with FP32 weights.
  *   - Closest-hit ties are broken by (t, prim_id, instance); a box is entered
  *     when its entry distance is <= the current tmax.
+ *   - Material textures are filtered in software (bilinear per level, linear
+ *     between levels at log2 of the footprint, point sampling for alpha);
+ *     the mip chains and the Lanczos resampling follow the reference's
+ *     kernels with binary32 arithmetic and a binary64 sinc.  These are pinned
+ *     by closed forms (tests/test_oracle_textures.py); parity with the
+ *     reference's hardware texture filtering is unpinned (CUDA does not
+ *     specify its filter weights).
  */
 #ifndef NOOR_ORACLE_H
 #define NOOR_ORACLE_H
