@@ -84,8 +84,8 @@ void noor_obj_free(noor_obj* obj);
 /* Reads an image as RGBA32F, row 0 first: Radiance RGBE (.hdr, flat or
  * run-length scanlines, -Y +X orientation), PFM (colour or grey, either byte
  * order, stored bottom row first), binary PNM (P5 grey, P6 colour, 8 or 16
- * bits: (v / max)^2.2), PNG (non-interlaced; every colour type and bit
- * depth; samples cut to 8 bits, then colour (v / 255)^2.2 and alpha v / 255,
+ * bits: (v / max)^2.2), PNG (plain or Adam7 interlaced; every colour type
+ * and bit depth; samples cut to 8 bits, then colour (v / 255)^2.2 and alpha v / 255,
  * as stb_image's loadf converts them).  Grey is replicated to rgb; alpha is 1
  * where the file has none.  The returned buffer is freed with
  * noor_image_free. */

@@ -593,7 +593,7 @@ noor_status load_pfm(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
 
 uint32_t be32(const unsigned char* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
 
-// PNG (ISO/IEC 15948): non-interlaced, every colour type and bit depth.
+// PNG (ISO/IEC 15948): every colour type and bit depth, plain or Adam7 interlaced.
 // Samples go to 8 bits first (16-bit ones keep their high byte), then to
 // float as stb_image's loadf does: colour (v / 255)^2.2, alpha v / 255.
 noor_status load_png(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
@@ -630,66 +630,93 @@ noor_status load_png(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
                           ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
     if (!chans || !depth_ok || w == 0 || h == 0 || w > 1 << 16 || h > 1 << 16)
         return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad header");
-    if (interlace) return fail(NOOR_ERR_UNSUPPORTED, "PNG: interlaced images are not supported");
+    if (interlace > 1) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad interlace method");
     if (ctype == 3 && (plte.empty() || plte.size() % 3)) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: no palette");
-    const size_t bits = (size_t)chans * depth;
-    const size_t stride = ((size_t)w * bits + 7) / 8, bpp = std::max<size_t>(1, bits / 8);
-    std::vector<unsigned char> raw((stride + 1) * h);
+    const size_t bits = (size_t)chans * depth, bpp = std::max<size_t>(1, bits / 8);
+    // passes: (x0, y0, dx, dy); one pass covering everything when not interlaced (Adam7 otherwise)
+    static const int adam7[7][4] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                                    {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+    static const int flat[1][4] = {{0, 0, 1, 1}};
+    const int npass = interlace ? 7 : 1;
+    const int (*passes)[4] = interlace ? adam7 : flat;
+    size_t total = 0;
+    for (int k = 0; k < npass; ++k) {
+        const uint32_t pw = w > (uint32_t)passes[k][0] ? (w - passes[k][0] + passes[k][2] - 1) / passes[k][2] : 0;
+        const uint32_t ph = h > (uint32_t)passes[k][1] ? (h - passes[k][1] + passes[k][3] - 1) / passes[k][3] : 0;
+        if (pw && ph) total += (((size_t)pw * bits + 7) / 8 + 1) * ph;
+    }
+    std::vector<unsigned char> raw(total);
     uLongf n = (uLongf)raw.size();
     if (uncompress(raw.data(), &n, idat.data(), (uLong)idat.size()) != Z_OK || n != raw.size())
         return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad image data");
-    // unfilter in place (filters 0-4: none, sub, up, average, Paeth)
-    std::vector<unsigned char> img(stride * h), prev(stride, 0);
-    for (uint32_t y = 0; y < h; ++y) {
-        const unsigned char f = raw[y * (stride + 1)];
-        const unsigned char* src = &raw[y * (stride + 1) + 1];
-        unsigned char* cur = &img[y * stride];
-        for (size_t i = 0; i < stride; ++i) {
-            const int a = i >= bpp ? cur[i - bpp] : 0, up = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
-            int v = src[i];
-            switch (f) {
-                case 0: break;
-                case 1: v += a; break;
-                case 2: v += up; break;
-                case 3: v += (a + up) >> 1; break;
-                case 4: {
-                    const int pp = a + up - c, pa = std::abs(pp - a), pb = std::abs(pp - up), pc = std::abs(pp - c);
-                    v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? up : c);
-                    break;
+    // 8-bit samples (palette indices for colour type 3) of every texel
+    std::vector<int> smp((size_t)w * h * chans);
+    size_t off = 0;
+    for (int k = 0; k < npass; ++k) {
+        const uint32_t x0 = passes[k][0], y0 = passes[k][1], dx = passes[k][2], dy = passes[k][3];
+        const uint32_t pw = w > x0 ? (w - x0 + dx - 1) / dx : 0, ph = h > y0 ? (h - y0 + dy - 1) / dy : 0;
+        if (!pw || !ph) continue;
+        const size_t stride = ((size_t)pw * bits + 7) / 8;
+        // unfilter (filters 0-4: none, sub, up, average, Paeth)
+        std::vector<unsigned char> cur(stride), prev(stride, 0);
+        for (uint32_t y = 0; y < ph; ++y) {
+            const unsigned char f = raw[off];
+            const unsigned char* src = &raw[off + 1];
+            off += stride + 1;
+            for (size_t i = 0; i < stride; ++i) {
+                const int a = i >= bpp ? cur[i - bpp] : 0, up = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+                int v = src[i];
+                switch (f) {
+                    case 0: break;
+                    case 1: v += a; break;
+                    case 2: v += up; break;
+                    case 3: v += (a + up) >> 1; break;
+                    case 4: {
+                        const int pp = a + up - c, pa = std::abs(pp - a), pb = std::abs(pp - up), pc = std::abs(pp - c);
+                        v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? up : c);
+                        break;
+                    }
+                    default: return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad filter");
                 }
-                default: return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad filter");
+                cur[i] = (unsigned char)v;
             }
-            cur[i] = (unsigned char)v;
+            for (uint32_t x = 0; x < pw; ++x)
+                for (int ch = 0; ch < chans; ++ch) {
+                    int v;
+                    if (depth == 16) {
+                        v = cur[2 * ((size_t)x * chans + ch)];  // the high byte
+                    } else if (depth == 8) {
+                        v = cur[(size_t)x * chans + ch];
+                    } else {
+                        const size_t bit = (size_t)x * depth;
+                        v = (cur[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+                        if (ctype != 3) v *= 255 / ((1 << depth) - 1);
+                    }
+                    smp[((size_t)(y0 + y * dy) * w + (x0 + x * dx)) * chans + ch] = v;
+                }
+            std::swap(cur, prev);
         }
-        std::memcpy(prev.data(), cur, stride);
     }
     W = w;
     H = h;
     out.assign(4 * (size_t)w * h, 1.0f);
-    auto sample = [&](const unsigned char* row, uint32_t x, int c) -> int {  // 8-bit value of sample c of texel x
-        if (depth == 16) return row[2 * ((size_t)x * chans + c)];
-        if (depth == 8) return row[(size_t)x * chans + c];
-        const size_t bit = (size_t)x * depth;
-        const int v = (row[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
-        return ctype == 3 ? v : v * (255 / ((1 << depth) - 1));
-    };
+    auto sample = [&](uint32_t y, uint32_t x, int c) -> int { return smp[((size_t)y * w + x) * chans + c]; };
     const float g = 2.2f;
     for (uint32_t y = 0; y < h; ++y) {
-        const unsigned char* row = &img[y * stride];
         for (uint32_t x = 0; x < w; ++x) {
             float* o = &out[4 * ((size_t)y * w + x)];
             if (ctype == 3) {
-                const int i = sample(row, x, 0);
+                const int i = sample(y, x, 0);
                 if ((size_t)i * 3 + 2 >= plte.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: palette index");
                 for (int c = 0; c < 3; ++c) o[c] = std::pow(plte[3 * i + c] / 255.0f, g);
                 o[3] = (size_t)i < trns.size() ? trns[i] / 255.0f : 1.0f;
             } else if (chans <= 2) {
-                const float v = std::pow(sample(row, x, 0) / 255.0f, g);
+                const float v = std::pow(sample(y, x, 0) / 255.0f, g);
                 o[0] = o[1] = o[2] = v;
-                if (chans == 2) o[3] = sample(row, x, 1) / 255.0f;
+                if (chans == 2) o[3] = sample(y, x, 1) / 255.0f;
             } else {
-                for (int c = 0; c < 3; ++c) o[c] = std::pow(sample(row, x, c) / 255.0f, g);
-                if (chans == 4) o[3] = sample(row, x, 3) / 255.0f;
+                for (int c = 0; c < 3; ++c) o[c] = std::pow(sample(y, x, c) / 255.0f, g);
+                if (chans == 4) o[3] = sample(y, x, 3) / 255.0f;
             }
         }
     }

@@ -257,13 +257,36 @@ def test_pfm_and_pnm(tmp_path, little):
                                atol=1e-7)
 
 
-def _png(path, arr, ctype, depth=8, palette=None, trns=None, seed=0):
+def _png(path, arr, ctype, depth=8, palette=None, trns=None, seed=0, interlace=False):
     """A PNG written here: rows packed at `depth`, each with a random filter
-    type (the filter arithmetic of ISO/IEC 15948 applied forward)."""
+    type (the filter arithmetic of ISO/IEC 15948 applied forward); with
+    `interlace`, the seven Adam7 sub-images one after the other."""
     import zlib
     rng = np.random.default_rng(seed)
     h, w = arr.shape[:2]
     chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    arr = arr.reshape(h, w, chans)
+    if interlace:
+        subs = [arr[y0::dy, x0::dx] for x0, y0, dx, dy in
+                ((0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2), (0, 1, 1, 2))]
+    else:
+        subs = [arr]
+    raw = b"".join(_png_rows(sub, chans, depth, rng) for sub in subs if sub.size)
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xffffffff)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0,
+                                                                 1 if interlace else 0))
+    if palette is not None:
+        png += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
+    if trns is not None:
+        png += chunk(b"tRNS", bytes(np.asarray(trns, np.uint8)))
+    png += chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b"")
+    with open(path, "wb") as fh:
+        fh.write(png)
+
+
+def _png_rows(arr, chans, depth, rng):
+    h, w = arr.shape[:2]
     a = arr.reshape(h, w * chans).astype(np.int64)
     if depth == 16:
         rows = [np.stack([r >> 8, r & 255], -1).reshape(-1) for r in a]
@@ -297,35 +320,28 @@ def _png(path, arr, ctype, depth=8, palette=None, trns=None, seed=0):
             e = r - np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, prev, ul))
         raw += bytes([f]) + bytes((e % 256).astype(np.uint8))
         prev = r
-    def chunk(t, d):
-        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xffffffff)
-    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
-    if palette is not None:
-        png += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
-    if trns is not None:
-        png += chunk(b"tRNS", bytes(np.asarray(trns, np.uint8)))
-    png += chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b"")
-    with open(path, "wb") as fh:
-        fh.write(png)
+    return raw
 
 
-@pytest.mark.parametrize("ctype,depth", [(2, 8), (6, 8), (0, 8), (4, 8), (2, 16), (0, 4), (0, 1), (3, 8), (3, 2)])
-def test_png(tmp_path, ctype, depth):
+@pytest.mark.parametrize("ctype,depth,interlace", [(2, 8, False), (6, 8, False), (0, 8, False), (4, 8, False),
+                                                   (2, 16, False), (0, 4, False), (0, 1, False), (3, 8, False),
+                                                   (3, 2, False), (6, 8, True), (0, 2, True), (3, 4, True)])
+def test_png(tmp_path, ctype, depth, interlace):
     rng = np.random.default_rng(ctype * 100 + depth)
-    h, w = 7, 13
+    h, w = (11, 14) if interlace else (7, 13)
     chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
     top = (1 << depth) - 1
     if ctype == 3:
         pal = rng.integers(0, 256, (1 << depth, 3))
         trns = rng.integers(0, 256, 1 << depth)
         idx = rng.integers(0, 1 << depth, (h, w))
-        _png(tmp_path / "p.png", idx, ctype, depth, pal, trns)
+        _png(tmp_path / "p.png", idx, ctype, depth, pal, trns, interlace=interlace)
         got = load_image(str(tmp_path / "p.png"))
         np.testing.assert_allclose(got[..., :3], (pal[idx] / 255.0) ** 2.2, rtol=1e-5, atol=1e-7)
         np.testing.assert_allclose(got[..., 3], trns[idx] / 255.0, rtol=1e-6)
         return
     arr = rng.integers(0, top + 1, (h, w, chans))
-    _png(tmp_path / "p.png", arr, ctype, depth)
+    _png(tmp_path / "p.png", arr, ctype, depth, interlace=interlace)
     got = load_image(str(tmp_path / "p.png"))
     v8 = arr >> 8 if depth == 16 else arr * (255 // top)
     col = (v8 / 255.0) ** 2.2
