@@ -11,8 +11,9 @@
  * The reference parses OBJ through the glm_obj library, which it does not
  * ship; the behaviour restated here is that library's published one
  * (glmReadOBJ, glmUnitize, glmFacetNormals, glmVertexNormals), see
- * noor_obj_load.  FBX (Autodesk SDK), EXR (tinyexr), JPEG and the other
- * 8-bit formats of stb_image besides PNG and PNM are not provided.
+ * noor_obj_load.  FBX (Autodesk SDK), JPEG and the other 8-bit formats of
+ * stb_image besides PNG and PNM, and the EXR compressions beyond ZIP are not
+ * provided.
  */
 #ifndef NOOR_LOADERS_H
 #define NOOR_LOADERS_H
@@ -86,7 +87,9 @@ void noor_obj_free(noor_obj* obj);
  * order, stored bottom row first), binary PNM (P5 grey, P6 colour, 8 or 16
  * bits: (v / max)^2.2), PNG (plain or Adam7 interlaced; every colour type
  * and bit depth; samples cut to 8 bits, then colour (v / 255)^2.2 and alpha v / 255,
- * as stb_image's loadf converts them).  Grey is replicated to rgb; alpha is 1
+ * as stb_image's loadf converts them), OpenEXR (single-part scanline; R G B
+ * A or Y channels in HALF or FLOAT; NONE, RLE, ZIPS or ZIP compression, as
+ * tinyexr reads them).  Grey is replicated to rgb; alpha is 1
  * where the file has none.  The returned buffer is freed with
  * noor_image_free. */
 noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba);

@@ -723,6 +723,157 @@ noor_status load_png(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     return NOOR_OK;
 }
 
+float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31, m = h & 1023;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = sign;
+        } else {  // subnormal: m * 2^-24
+            const float f = std::ldexp((float)m, -24);
+            std::memcpy(&bits, &f, 4);
+            bits |= sign;
+        }
+    } else if (e == 31) {
+        bits = sign | 0x7F800000u | (m << 13);
+    } else {
+        bits = sign | ((e + 112) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+// OpenEXR (the reference reads .exr through tinyexr, noor/source/texture.cpp:51-56):
+// single-part scanline images, channels R G B (A, or Y for grey) in HALF or
+// FLOAT, compression NONE, RLE, ZIPS or ZIP.
+noor_status load_exr(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
+    // version flags: tiled (bit 9), deep data (bit 11), multi-part (bit 12)
+    if (b.size() < 8 || (b[5] & 0x1A)) return fail(NOOR_ERR_UNSUPPORTED, "EXR: only single-part scanline files");
+    size_t p = 8;
+    struct Ch { std::string name; int type; };
+    std::vector<Ch> chans;
+    int comp = -1;
+    int32_t win[4] = {0, 0, -1, -1};
+    auto rd32 = [&](size_t at) { int32_t v; std::memcpy(&v, &b[at], 4); return v; };
+    for (;;) {
+        if (p >= b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: truncated header");
+        if (b[p] == 0) { ++p; break; }
+        std::string name, type;
+        while (p < b.size() && b[p]) name += (char)b[p++];
+        ++p;
+        while (p < b.size() && b[p]) type += (char)b[p++];
+        ++p;
+        if (p + 4 > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: truncated header");
+        const int32_t size = rd32(p);
+        p += 4;
+        if (size < 0 || p + (size_t)size > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad attribute");
+        const size_t end = p + (size_t)size;
+        if (name == "channels") {
+            size_t q = p;
+            while (q < end && b[q]) {
+                Ch c;
+                while (q < end && b[q]) c.name += (char)b[q++];
+                ++q;
+                if (q + 16 > end) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad channel list");
+                c.type = rd32(q);
+                if (rd32(q + 8) != 1 || rd32(q + 12) != 1) return fail(NOOR_ERR_UNSUPPORTED, "EXR: subsampled channels");
+                q += 16;
+                chans.push_back(c);
+            }
+        } else if (name == "compression" && size >= 1) {
+            comp = b[p];
+        } else if (name == "dataWindow" && size >= 16) {
+            for (int k = 0; k < 4; ++k) win[k] = rd32(p + 4 * k);
+        }
+        p = end;
+    }
+    const int64_t w = (int64_t)win[2] - win[0] + 1, h = (int64_t)win[3] - win[1] + 1;
+    if (chans.empty() || w <= 0 || h <= 0 || w > 1 << 16 || h > 1 << 16)
+        return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad header");
+    if (comp < 0 || comp > 3) return fail(NOOR_ERR_UNSUPPORTED, "EXR: compression other than NONE, RLE, ZIPS, ZIP");
+    size_t line_bytes = 0;
+    for (const Ch& c : chans) {
+        if (c.type != 1 && c.type != 2) return fail(NOOR_ERR_UNSUPPORTED, "EXR: only HALF and FLOAT channels");
+        line_bytes += (size_t)w * (c.type == 1 ? 2 : 4);
+    }
+    const int lines = comp == 3 ? 16 : 1;
+    const size_t nchunks = (size_t)((h + lines - 1) / lines);
+    if (p + 8 * nchunks > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: truncated offsets");
+    W = (uint32_t)w;
+    H = (uint32_t)h;
+    out.assign(4 * (size_t)W * H, 0.0f);
+    for (size_t i = 0; i < (size_t)W * H; ++i) out[4 * i + 3] = 1.0f;
+    std::vector<unsigned char> buf, tmp;
+    for (size_t k = 0; k < nchunks; ++k) {
+        uint64_t off;
+        std::memcpy(&off, &b[p + 8 * k], 8);
+        if (off + 8 > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad chunk offset");
+        const int32_t y0 = rd32((size_t)off) - win[1], size = rd32((size_t)off + 4);
+        if (y0 < 0 || y0 >= h || size < 0 || off + 8 + (uint64_t)size > b.size())
+            return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad chunk");
+        const int ny = (int)std::min<int64_t>(lines, h - y0);
+        const size_t want = line_bytes * ny;
+        const unsigned char* data = &b[(size_t)off + 8];
+        if (comp == 0 || (size_t)size == want) {  // stored uncompressed (also when compression did not pay)
+            if ((size_t)size != want) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad chunk size");
+            buf.assign(data, data + want);
+        } else {
+            tmp.assign(want, 0);
+            if (comp == 1) {  // RLE: count < 0 -> -count literal bytes, else count + 1 copies
+                size_t q = 0, o = 0;
+                while (q < (size_t)size && o < want) {
+                    const int c = (signed char)data[q++];
+                    if (c < 0) {
+                        if (q + (size_t)(-c) > (size_t)size || o + (size_t)(-c) > want)
+                            return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad RLE");
+                        std::memcpy(&tmp[o], &data[q], (size_t)(-c));
+                        q += (size_t)(-c);
+                        o += (size_t)(-c);
+                    } else {
+                        if (q >= (size_t)size || o + (size_t)c + 1 > want) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad RLE");
+                        std::memset(&tmp[o], data[q++], (size_t)c + 1);
+                        o += (size_t)c + 1;
+                    }
+                }
+                if (o != want) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: short RLE chunk");
+            } else {
+                uLongf n = (uLongf)want;
+                if (uncompress(tmp.data(), &n, data, (uLong)size) != Z_OK || n != want)
+                    return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad ZIP chunk");
+            }
+            // undo the predictor, then the split into even and odd bytes
+            for (size_t i = 1; i < want; ++i) tmp[i] = (unsigned char)(tmp[i - 1] + tmp[i] - 128);
+            buf.resize(want);
+            const size_t half = (want + 1) / 2;
+            for (size_t i = 0; i < want; ++i) buf[i] = (i & 1) ? tmp[half + i / 2] : tmp[i / 2];
+        }
+        size_t q = 0;
+        for (int yy = 0; yy < ny; ++yy) {
+            float* row = &out[4 * (size_t)(y0 + yy) * W];
+            for (const Ch& c : chans) {
+                const int slot = c.name == "R" ? 0 : c.name == "G" ? 1 : c.name == "B" ? 2 : c.name == "A" ? 3
+                                                                                               : c.name == "Y" ? 4 : -1;
+                for (uint32_t x = 0; x < W; ++x) {
+                    float v;
+                    if (c.type == 1) {
+                        uint16_t hv;
+                        std::memcpy(&hv, &buf[q], 2);
+                        q += 2;
+                        v = half_to_float(hv);
+                    } else {
+                        std::memcpy(&v, &buf[q], 4);
+                        q += 4;
+                    }
+                    if (slot == 4) row[4 * x] = row[4 * x + 1] = row[4 * x + 2] = v;
+                    else if (slot >= 0) row[4 * x + slot] = v;
+                }
+            }
+        }
+    }
+    return NOOR_OK;
+}
+
 }  // namespace
 
 noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba) {
@@ -737,6 +888,7 @@ noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height,
     else if (b.size() >= 2 && b[0] == 'P' && (b[1] == '5' || b[1] == '6')) s = load_pnm(b, W, H, px);
     else if (b.size() >= 2 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) s = load_pfm(b, W, H, px);
     else if (b.size() >= 8 && std::memcmp(b.data(), "\x89PNG\r\n\x1a\n", 8) == 0) s = load_png(b, W, H, px);
+    else if (b.size() >= 8 && b[0] == 0x76 && b[1] == 0x2f && b[2] == 0x31 && b[3] == 0x01) s = load_exr(b, W, H, px);
     else s = fail(NOOR_ERR_UNSUPPORTED, std::string("unsupported image format: ") + path);
     if (s != NOOR_OK) return s;
     float* o = new float[px.size()];

@@ -353,6 +353,86 @@ def test_png(tmp_path, ctype, depth, interlace):
     np.testing.assert_allclose(got[..., 3], alpha, rtol=1e-6)
 
 
+def _exr(path, img, comp, half):
+    """A scanline OpenEXR file written here: channels in name order, each
+    line channel by channel; ZIP (16 lines) and ZIPS (1 line) chunks split
+    into even and odd bytes and delta-coded before zlib."""
+    import zlib
+    h, w, c = img.shape
+    names = ["R", "G", "B", "A"][:c] if c > 1 else ["Y"]
+    order = sorted(range(len(names)), key=lambda k: names[k])
+    ptype = 1 if half else 2
+    def attr(name, typ, data):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(data)) + data
+    chl = b"".join(names[k].encode() + b"\0" + struct.pack("<iB3xii", ptype, 0, 1, 1) for k in order) + b"\0"
+    hdr = (b"\x76\x2f\x31\x01" + struct.pack("<BBBB", 2, 0, 0, 0) + attr("channels", "chlist", chl)
+           + attr("compression", "compression", bytes([comp]))
+           + attr("dataWindow", "box2i", struct.pack("<4i", 0, 0, w - 1, h - 1))
+           + attr("displayWindow", "box2i", struct.pack("<4i", 0, 0, w - 1, h - 1))
+           + attr("lineOrder", "lineOrder", b"\0") + attr("pixelAspectRatio", "float", struct.pack("<f", 1.0))
+           + attr("screenWindowCenter", "v2f", struct.pack("<2f", 0, 0))
+           + attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0")
+    lines = 16 if comp == 3 else 1
+    dt = "<f2" if half else "<f4"
+    chunks = []
+    for y0 in range(0, h, lines):
+        raw = b"".join(img[y, :, k].astype(dt).tobytes() for y in range(y0, min(h, y0 + lines)) for k in order)
+        data = raw
+        if comp in (1, 2, 3):
+            a = np.frombuffer(raw, np.uint8).astype(np.int64)
+            t = np.concatenate([a[0::2], a[1::2]])
+            d = t.copy()
+            d[1:] = (t[1:] - t[:-1] + 128) % 256
+            if comp == 1:  # runs of >= 3 equal bytes: (n - 1, value); literals: (-n, bytes)
+                z, i, n = bytearray(), 0, len(d)
+                while i < n:
+                    j = i
+                    while j < n and j - i < 128 and d[j] == d[i]:
+                        j += 1
+                    if j - i >= 3:
+                        z += bytes([j - i - 1, int(d[i])])
+                        i = j
+                        continue
+                    k = i
+                    while k < n and k - i < 127 and not (k + 2 < n and d[k] == d[k + 1] == d[k + 2]):
+                        k += 1
+                    z += bytes([(256 - (k - i)) % 256]) + bytes(d[i:k].astype(np.uint8))
+                    i = k
+                z = bytes(z)
+            else:
+                z = zlib.compress(bytes(d.astype(np.uint8)))
+            data = z if len(z) < len(raw) else raw
+        chunks.append(struct.pack("<ii", y0, len(data)) + data)
+    table_end = len(hdr) + 8 * len(chunks)
+    offs, pos = [], table_end
+    for ch in chunks:
+        offs.append(pos)
+        pos += len(ch)
+    with open(path, "wb") as f:
+        f.write(hdr + struct.pack(f"<{len(offs)}Q", *offs) + b"".join(chunks))
+
+
+@pytest.mark.parametrize("comp,half,chans", [(0, True, 3), (0, False, 4), (1, True, 3), (2, True, 4), (3, True, 3),
+                                             (3, False, 3), (3, True, 1)])
+def test_openexr(tmp_path, comp, half, chans):
+    rng = np.random.default_rng(comp * 10 + chans)
+    h, w = 37, 23
+    img = (rng.standard_normal((h, w, chans)) * 10 ** rng.uniform(-3, 3, (h, w, 1))).astype(np.float32)
+    img[0, :4] = 0.0
+    img[2, :] = 1.5                          # runs
+    img[1, :3, 0] = [1e-7, -6e-8, 65504.0]   # half subnormals and the largest half
+    if half:
+        img = img.astype(np.float16).astype(np.float32)
+    _exr(tmp_path / "x.exr", img, comp, half)
+    got = load_image(str(tmp_path / "x.exr"))
+    want = np.ones((h, w, 4), np.float32)
+    if chans == 1:
+        want[..., :3] = img
+    else:
+        want[..., :chans] = img
+    np.testing.assert_array_equal(got, want)
+
+
 def test_bad_files_fail_cleanly(tmp_path):
     _write(tmp_path / "bad.obj", "v 0 0 0\nf 1 2 3\n")
     with pytest.raises(abi.NoorError):
