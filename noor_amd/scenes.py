@@ -523,7 +523,8 @@ def quad_uv(corner, u, v):
 
 
 def textured(seed=8, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, lens_radius=0.0,
-             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL, tex=64, flat=False, env_size=(128, 64)):
+             rng_kind=abi.RNG_PCG32, compat=abi.COMPAT_ALL, tex=64, flat=False, env_size=(128, 64),
+             camera_type=abi.CAMERA_PERSP):
     """The image-texture pipeline of the reference: mip-mapped colour,
     roughness and emittance textures read at the pixel footprint (ray
     differentials from the camera, carried through mirror, metal and glass
@@ -585,7 +586,7 @@ def textured(seed=8, width=96, height=80, spp=4, bounces=6, rr=3, mis=False, len
     sb.set_env(env_map(env_size[0], env_size[1], seed + 3, lobes=2, sigma_deg=8.0, peak=20.0))
     payload = sb.build()
     cam = make_camera((0.0, 0.0, 0.95), (0.0, -0.15, -1.0), (0, 1, 0), np.radians(60.0), width, height,
-                      lens_radius=lens_radius)
+                      lens_radius=lens_radius, camera_type=camera_type, ortho_zoom=1.2)
     spec = make_spec(payload, bounces=bounces, rr=rr, skydome_type=abi.SKY_HDR, mis=mis)
     opt = make_options(rng=rng_kind, compat=compat)
     return Workload("textured", payload, cam, spec, opt, None, spp, info=dict(triangles=payload.view.num_triangles))

@@ -66,6 +66,22 @@ def test_batch_with_small_pool_and_lds_split(wl):
         np.testing.assert_array_equal(r.batch_film(i), m.film())
 
 
+@pytest.mark.parametrize("mis", [False, True])
+def test_textured_batch_with_mis(mis):
+    """Batches of the textured scene (per-view camera differentials, bump,
+    alpha, textures), with and without MIS."""
+    tw = scenes.textured(mis=mis)
+    cams = [make_camera((0.0, 0.0, 0.95), (0.0, -0.15, -1.0), fov=np.radians(60.0), w=48, h=40),
+            make_camera((0.3, 0.2, 0.8), (-0.2, -0.3, -1.0), fov=np.radians(45.0), w=32, h=24, lens_radius=0.03),
+            make_camera((-0.2, 0.1, 0.9), (0.1, -0.2, -1.0), w=20, h=30, camera_type=abi.CAMERA_ORTHO)]
+    r = Renderer.from_workload(tw)
+    r.render_batch([(c, 1, 2, 0) for c in cams])
+    for i, c in enumerate(cams):
+        m = Renderer(tw.payload, c, tw.spec, tw.options, tw.hosek)
+        m.render(spp=2, frame=1)
+        np.testing.assert_array_equal(r.batch_film(i), m.film())
+
+
 def test_autotune_picks_a_measured_setting(wl):
     cams = _cams()
     r = Renderer.from_workload(wl)

@@ -53,6 +53,24 @@ def test_resized_textures_vs_oracle(compat):
     assert err <= TOL_IMAGE
 
 
+@pytest.mark.parametrize("camera", [abi.CAMERA_ORTHO, abi.CAMERA_ENV])
+def test_textured_other_cameras_vs_oracle(camera):
+    """Orthographic (auxiliary origins one pixel apart) and environment
+    (auxiliary directions one pixel apart in angle) camera differentials."""
+    wl = scenes.textured(camera_type=camera)
+    r = Renderer.from_workload(wl)
+    r.render(spp=2)
+    g = r.film()
+    c, _ = OracleScene.from_workload(wl).render(1, 2)
+    err = mean_rel_err(g, c)
+    print(f"camera={camera}: megakernel vs oracle {err:.3e}, identical {np.mean(np.all(g == c, axis=-1)):.4f}",
+          flush=True)
+    assert err <= TOL_IMAGE and float(g[..., :3].mean()) > 0.0
+    w = Renderer.from_workload(wl)
+    w.render_wavefront(spp=2)
+    np.testing.assert_array_equal(w.film(), g)
+
+
 def test_alpha_cut_hit_queries_bit_exact():
     """Closest and any hit through the alpha-cut panel of the textured scene."""
     wl = scenes.textured()
