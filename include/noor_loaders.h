@@ -11,9 +11,8 @@
  * The reference parses OBJ through the glm_obj library, which it does not
  * ship; the behaviour restated here is that library's published one
  * (glmReadOBJ, glmUnitize, glmFacetNormals, glmVertexNormals), see
- * noor_obj_load.  FBX (Autodesk SDK), JPEG and the other 8-bit formats of
- * stb_image besides PNG and PNM, and the EXR compressions beyond ZIP are not
- * provided.
+ * noor_obj_load.  FBX (Autodesk SDK), JPEG, BMP, PSD and GIF (stb_image),
+ * and the EXR compressions beyond ZIP are not provided.
  */
 #ifndef NOOR_LOADERS_H
 #define NOOR_LOADERS_H
@@ -87,7 +86,9 @@ void noor_obj_free(noor_obj* obj);
  * order, stored bottom row first), binary PNM (P5 grey, P6 colour, 8 or 16
  * bits: (v / max)^2.2), PNG (plain or Adam7 interlaced; every colour type
  * and bit depth; samples cut to 8 bits, then colour (v / 255)^2.2 and alpha v / 255,
- * as stb_image's loadf converts them), OpenEXR (single-part scanline; R G B
+ * as stb_image's loadf converts them), TGA (by the .tga extension;
+ * uncompressed or run-length true colour, grey or colour-mapped; converted
+ * the same way), OpenEXR (single-part scanline; R G B
  * A or Y channels in HALF or FLOAT; NONE, RLE, ZIPS or ZIP compression, as
  * tinyexr reads them).  Grey is replicated to rgb; alpha is 1
  * where the file has none.  The returned buffer is freed with

@@ -723,6 +723,87 @@ noor_status load_png(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     return NOOR_OK;
 }
 
+// Truevision TGA: uncompressed or run-length, true-colour (24/32 bit), grey
+// (8 bit) or colour-mapped (8-bit indices into a 24/32-bit map); converted
+// as stb_image's loadf does (colour (v / 255)^2.2, alpha v / 255).
+noor_status load_tga(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
+    if (b.size() < 18) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: truncated header");
+    const int idlen = b[0], cmtype = b[1], itype = b[2];
+    const int cmfirst = b[3] | b[4] << 8, cmlen = b[5] | b[6] << 8, cmbits = b[7];
+    const int w = b[12] | b[13] << 8, h = b[14] | b[15] << 8, bits = b[16], desc = b[17];
+    const bool rle = itype >= 9;
+    const int base = rle ? itype - 8 : itype;
+    if (!(base == 1 || base == 2 || base == 3) || w == 0 || h == 0)
+        return fail(NOOR_ERR_UNSUPPORTED, "TGA: only colour-mapped, true-colour and grey images");
+    const bool mapped = base == 1;
+    if ((mapped && (cmtype != 1 || bits != 8 || (cmbits != 24 && cmbits != 32))) ||
+        (base == 2 && bits != 24 && bits != 32) || (base == 3 && bits != 8))
+        return fail(NOOR_ERR_UNSUPPORTED, "TGA: unsupported pixel depth");
+    size_t p = 18 + (size_t)idlen;
+    std::vector<unsigned char> cmap;
+    if (cmtype == 1) {
+        const size_t n = (size_t)cmlen * ((cmbits + 7) / 8);
+        if (p + n > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: truncated colour map");
+        cmap.assign(b.begin() + p, b.begin() + p + n);
+        p += n;
+    }
+    const int bpp = bits / 8, cbpp = cmbits / 8;
+    std::vector<unsigned char> px((size_t)w * h * bpp);
+    if (!rle) {
+        if (p + px.size() > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: truncated pixels");
+        std::memcpy(px.data(), &b[p], px.size());
+    } else {
+        size_t o = 0;
+        while (o < px.size()) {
+            if (p >= b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: truncated run");
+            const int c = b[p++];
+            const size_t n = (size_t)(c & 127) + 1;
+            if (o + n * bpp > px.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: run overflows the image");
+            if (c & 128) {
+                if (p + bpp > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: truncated run");
+                for (size_t k = 0; k < n; ++k) std::memcpy(&px[o + k * bpp], &b[p], bpp);
+                p += bpp;
+            } else {
+                if (p + n * bpp > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: truncated run");
+                std::memcpy(&px[o], &b[p], n * bpp);
+                p += n * bpp;
+            }
+            o += n * bpp;
+        }
+    }
+    W = (uint32_t)w;
+    H = (uint32_t)h;
+    out.assign(4 * (size_t)w * h, 1.0f);
+    const bool top = (desc & 0x20) != 0, right = (desc & 0x10) != 0;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const unsigned char* s = &px[((size_t)y * w + x) * bpp];
+            const int oy = top ? y : h - 1 - y, ox = right ? w - 1 - x : x;
+            float* o = &out[4 * ((size_t)oy * w + ox)];
+            int r, g, bl, a = 255, ch;
+            if (mapped) {
+                const int i = s[0] - cmfirst;
+                if (i < 0 || (size_t)(i + 1) * cbpp > cmap.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: colour index");
+                const unsigned char* c = &cmap[(size_t)i * cbpp];
+                bl = c[0], g = c[1], r = c[2];
+                if (cbpp == 4) a = c[3];
+                ch = cbpp;
+            } else if (base == 3) {
+                r = g = bl = s[0];
+                ch = 1;
+            } else {
+                bl = s[0], g = s[1], r = s[2];
+                if (bpp == 4) a = s[3];
+                ch = bpp;
+            }
+            o[0] = std::pow(r / 255.0f, 2.2f);
+            o[1] = std::pow(g / 255.0f, 2.2f);
+            o[2] = std::pow(bl / 255.0f, 2.2f);
+            if (ch == 4) o[3] = a / 255.0f;
+        }
+    return NOOR_OK;
+}
+
 float half_to_float(uint16_t h) {
     const uint32_t sign = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31, m = h & 1023;
     uint32_t bits;
@@ -889,6 +970,11 @@ noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height,
     else if (b.size() >= 2 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) s = load_pfm(b, W, H, px);
     else if (b.size() >= 8 && std::memcmp(b.data(), "\x89PNG\r\n\x1a\n", 8) == 0) s = load_png(b, W, H, px);
     else if (b.size() >= 8 && b[0] == 0x76 && b[1] == 0x2f && b[2] == 0x31 && b[3] == 0x01) s = load_exr(b, W, H, px);
+    else if (b.size() >= 18 && ((b[2] >= 1 && b[2] <= 3) || (b[2] >= 9 && b[2] <= 11)) && b[1] <= 1 &&
+             std::string(path).size() >= 4 &&
+             (std::string(path).compare(std::string(path).size() - 4, 4, ".tga") == 0 ||
+              std::string(path).compare(std::string(path).size() - 4, 4, ".TGA") == 0))
+        s = load_tga(b, W, H, px);  // TGA has no signature: by extension
     else s = fail(NOOR_ERR_UNSUPPORTED, std::string("unsupported image format: ") + path);
     if (s != NOOR_OK) return s;
     float* o = new float[px.size()];

@@ -433,6 +433,68 @@ def test_openexr(tmp_path, comp, half, chans):
     np.testing.assert_array_equal(got, want)
 
 
+def _tga(path, img, kind, rle=False, top=False, cmap=None):
+    """A TGA written here: kind 2 true colour (BGR or BGRA), 3 grey, 1
+    colour-mapped (8-bit indices); run-length packets of up to 128 texels."""
+    h, w = img.shape[:2]
+    if kind == 2:
+        bpp = img.shape[2]
+        px = img[..., [2, 1, 0, 3][:bpp]]
+    else:
+        bpp = 1
+        px = img.reshape(h, w, 1)
+    rows = px if top else px[::-1]
+    flat = rows.reshape(-1, bpp).astype(np.uint8)
+    if rle:
+        body, i, n = bytearray(), 0, len(flat)
+        while i < n:
+            j = i
+            while j < n and j - i < 128 and np.array_equal(flat[j], flat[i]):
+                j += 1
+            if j - i >= 2:
+                body += bytes([127 + (j - i)]) + bytes(flat[i])
+                i = j
+            else:
+                k = i
+                while k < n and k - i < 128 and not (k + 1 < n and np.array_equal(flat[k], flat[k + 1])):
+                    k += 1
+                body += bytes([k - i - 1]) + bytes(flat[i:k].reshape(-1))
+                i = k
+        body = bytes(body)
+    else:
+        body = bytes(flat.reshape(-1))
+    cm = b""
+    if kind == 1:
+        cm = bytes(np.asarray(cmap, np.uint8)[:, [2, 1, 0]].reshape(-1))
+    hdr = struct.pack("<BBBHHBHHHHBB", 0, 1 if kind == 1 else 0, kind + (8 if rle else 0), 0,
+                      len(cmap) if kind == 1 else 0, 24 if kind == 1 else 0, 0, 0, w, h, bpp * 8,
+                      (0x20 if top else 0) | (8 if bpp == 4 else 0))
+    with open(path, "wb") as f:
+        f.write(hdr + cm + body)
+
+
+@pytest.mark.parametrize("kind,chans,rle,top", [(2, 3, False, False), (2, 4, True, True), (3, 1, True, False),
+                                                (1, 1, False, True), (1, 1, True, False)])
+def test_tga(tmp_path, kind, chans, rle, top):
+    rng = np.random.default_rng(kind * 10 + chans)
+    h, w = 6, 9
+    if kind == 1:
+        pal = rng.integers(0, 256, (16, 3))
+        idx = rng.integers(0, 16, (h, w))
+        idx[2, :] = 5
+        _tga(tmp_path / "t.tga", idx, 1, rle, top, pal)
+        np.testing.assert_allclose(load_image(str(tmp_path / "t.tga"))[..., :3], (pal[idx] / 255.0) ** 2.2,
+                                   rtol=1e-5, atol=1e-7)
+        return
+    img = rng.integers(0, 256, (h, w, chans))
+    img[1, :] = img[1, 0]  # runs
+    _tga(tmp_path / "t.tga", img if chans > 1 else img[..., 0], kind, rle, top)
+    got = load_image(str(tmp_path / "t.tga"))
+    col = (img[..., :3] / 255.0) ** 2.2 if chans > 1 else np.repeat((img / 255.0) ** 2.2, 3, -1)
+    np.testing.assert_allclose(got[..., :3], col, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(got[..., 3], img[..., 3] / 255.0 if chans == 4 else 1.0, rtol=1e-6)
+
+
 def test_bad_files_fail_cleanly(tmp_path):
     _write(tmp_path / "bad.obj", "v 0 0 0\nf 1 2 3\n")
     with pytest.raises(abi.NoorError):
