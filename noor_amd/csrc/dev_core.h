@@ -1173,6 +1173,9 @@ __device__ __forceinline__ void hit_differential(const Isect& I, const RayDiff& 
              hd.g.dvdy);
 }
 
+// The hit record (CudaIntersection::updateIntersection, noor/cuda/intersection.cuh:317-397).
+// TEX: also the derivatives of the ray differentials `rd` at the hit into `hd`
+// (zero when the ray has none), and an emitter keeps its barycentrics as uv.
 template <bool TEX = false>
 __device__ __forceinline__ void make_isect(const DevScene& S, const Ray& ray, const Hit& h, Isect& I,
                                            const RayDiff* rd = nullptr, HitDiff* hd = nullptr) {
