@@ -11,8 +11,8 @@
  * The reference parses OBJ through the glm_obj library, which it does not
  * ship; the behaviour restated here is that library's published one
  * (glmReadOBJ, glmUnitize, glmFacetNormals, glmVertexNormals), see
- * noor_obj_load.  FBX (Autodesk SDK), JPEG, BMP, PSD and GIF (stb_image),
- * and the EXR compressions beyond ZIP are not provided.
+ * noor_obj_load.  FBX (Autodesk SDK), progressive JPEG, BMP, PSD and GIF
+ * (stb_image), and the EXR compressions beyond ZIP are not provided.
  */
 #ifndef NOOR_LOADERS_H
 #define NOOR_LOADERS_H
@@ -90,7 +90,9 @@ void noor_obj_free(noor_obj* obj);
  * uncompressed or run-length true colour, grey or colour-mapped; converted
  * the same way), OpenEXR (single-part scanline; R G B
  * A or Y channels in HALF or FLOAT; NONE, RLE, ZIPS or ZIP compression, as
- * tinyexr reads them).  Grey is replicated to rgb; alpha is 1
+ * tinyexr reads them), baseline JPEG (sequential Huffman, 8-bit, grey or
+ * YCbCr with sampling factors 1 or 2, restart intervals; chroma upsampled by
+ * replication; converted as PNG).  Grey is replicated to rgb; alpha is 1
  * where the file has none.  The returned buffer is freed with
  * noor_image_free. */
 noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba);

@@ -804,6 +804,259 @@ noor_status load_tga(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     return NOOR_OK;
 }
 
+// Baseline JPEG (ITU T.81 sequential DCT, Huffman, 8-bit samples): one
+// (grey) or three (YCbCr, JFIF) components, sampling factors 1 or 2, restart
+// intervals.  Chroma is upsampled by replication (stb_image interpolates
+// 2x chroma; the two differ by at most the chroma step of one texel).
+// Converted as stb_image's loadf does: (v / 255)^2.2.
+struct JpegHuff {
+    uint8_t bits[17] = {0}, vals[256] = {0};
+    int mincode[17] = {0}, maxcode[18] = {0}, valptr[17] = {0};
+    bool ok = false;
+};
+
+struct JpegBits {
+    const std::vector<unsigned char>& b;
+    size_t p;
+    uint32_t acc = 0;
+    int n = 0;
+    bool marker = false;
+    int starved = 0;  // bytes fed as zeros since the last restart (the scan ran short)
+    JpegBits(const std::vector<unsigned char>& buf, size_t pos) : b(buf), p(pos) {}
+    int bit() {
+        if (n == 0) {
+            unsigned char c = 0;
+            if (marker || p >= b.size()) ++starved;
+            if (!marker && p < b.size()) {
+                c = b[p++];
+                if (c == 0xFF) {
+                    const unsigned char d = p < b.size() ? b[p] : 0;
+                    if (d == 0) ++p;           // stuffed zero
+                    else { marker = true; c = 0; --p; ++starved; }  // a marker: feed zeros
+                }
+            }
+            acc = c;
+            n = 8;
+        }
+        --n;
+        return (acc >> n) & 1;
+    }
+    int bits(int k) {
+        int v = 0;
+        for (int i = 0; i < k; ++i) v = (v << 1) | bit();
+        return v;
+    }
+    void reset() {  // at a restart marker
+        n = 0;
+        marker = false;
+        starved = 0;
+        while (p + 1 < b.size() && !(b[p] == 0xFF && b[p + 1] >= 0xD0 && b[p + 1] <= 0xD7)) ++p;
+        if (p + 1 < b.size()) p += 2;
+    }
+};
+
+static int jpeg_decode_huff(JpegBits& bs, const JpegHuff& h) {
+    int code = 0;
+    for (int l = 1; l <= 16; ++l) {
+        code = (code << 1) | bs.bit();
+        if (h.maxcode[l] >= 0 && code <= h.maxcode[l] && code >= h.mincode[l])
+            return h.vals[h.valptr[l] + code - h.mincode[l]];
+    }
+    return -1;
+}
+
+static int jpeg_extend(int v, int t) { return t && v < (1 << (t - 1)) ? v - (1 << t) + 1 : v; }
+
+noor_status load_jpeg(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
+    static const int zz[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,
+                               12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6,  7,  14, 21, 28,
+                               35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51,
+                               58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+    uint16_t qt[4][64] = {{0}};
+    JpegHuff hd[4], ha[4];
+    double idct[8][8];  // C(u) cos((2x + 1) u pi / 16)
+    for (int x = 0; x < 8; ++x)
+        for (int u = 0; u < 8; ++u)
+            idct[x][u] = (u ? 1.0 : 0.70710678118654752) * std::cos((2 * x + 1) * u * 3.14159265358979323846 / 16);
+    struct Comp { int id, h, v, tq, td = 0, ta = 0; std::vector<uint8_t> px; int bw = 0, bh = 0; };
+    std::vector<Comp> comps;
+    int w = 0, h = 0, restart = 0;
+    size_t p = 2;
+    auto u16 = [&](size_t at) { return (int)b[at] << 8 | b[at + 1]; };
+    while (p + 4 <= b.size()) {
+        if (b[p] != 0xFF) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: expected a marker");
+        const int m = b[p + 1];
+        if (m == 0xD8 || (m >= 0xD0 && m <= 0xD7) || m == 0x01 || m == 0xFF) { p += (m == 0xFF) ? 1 : 2; continue; }
+        if (m == 0xD9) break;
+        const int len = u16(p + 2);
+        if (len < 2 || p + 2 + (size_t)len > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad segment");
+        const size_t q0 = p + 4, q1 = p + 2 + len;
+        if (m == 0xC2 || m == 0xC6 || m == 0xCA || m == 0xCE || (m >= 0xC3 && m <= 0xCF && m != 0xC4 && m != 0xC8 && m != 0xCC))
+            return fail(NOOR_ERR_UNSUPPORTED, "JPEG: only baseline (sequential Huffman) images");
+        if (m == 0xDB) {  // quantisation tables
+            size_t q = q0;
+            while (q < q1) {
+                const int pq = b[q] >> 4, tq = b[q] & 3;
+                ++q;
+                for (int i = 0; i < 64; ++i) {
+                    if (q + (pq ? 2 : 1) > q1) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad DQT");
+                    qt[tq][zz[i]] = pq ? (uint16_t)u16(q) : b[q];
+                    q += pq ? 2 : 1;
+                }
+            }
+        } else if (m == 0xC4) {  // Huffman tables
+            size_t q = q0;
+            while (q < q1) {
+                const int tc = b[q] >> 4, th = b[q] & 3;
+                JpegHuff& t = tc ? ha[th] : hd[th];
+                ++q;
+                int total = 0;
+                for (int l = 1; l <= 16; ++l) {
+                    if (q >= q1) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad DHT");
+                    t.bits[l] = b[q++];
+                    total += t.bits[l];
+                }
+                if (total > 256 || q + total > q1) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad DHT");
+                for (int i = 0; i < total; ++i) t.vals[i] = b[q++];
+                int code = 0, k = 0;
+                for (int l = 1; l <= 16; ++l) {
+                    t.valptr[l] = k;
+                    t.mincode[l] = code;
+                    code += t.bits[l];
+                    k += t.bits[l];
+                    t.maxcode[l] = t.bits[l] ? code - 1 : -1;
+                    code <<= 1;
+                }
+                t.ok = true;
+            }
+        } else if (m == 0xDD) {
+            restart = u16(q0);
+        } else if (m == 0xC0 || m == 0xC1) {
+            if (b[q0] != 8) return fail(NOOR_ERR_UNSUPPORTED, "JPEG: only 8-bit samples");
+            h = u16(q0 + 1);
+            w = u16(q0 + 3);
+            const int nc = b[q0 + 5];
+            if ((nc != 1 && nc != 3) || q0 + 6 + 3 * (size_t)nc > q1 || w == 0 || h == 0)
+                return fail(NOOR_ERR_UNSUPPORTED, "JPEG: only grey or three-component images");
+            if ((uint64_t)w * h > (1ull << 28)) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: image too large");
+            for (int i = 0; i < nc; ++i) {
+                Comp c;
+                c.id = b[q0 + 6 + 3 * i];
+                c.h = b[q0 + 7 + 3 * i] >> 4;
+                c.v = b[q0 + 7 + 3 * i] & 15;
+                c.tq = b[q0 + 8 + 3 * i] & 3;
+                if (c.h < 1 || c.h > 2 || c.v < 1 || c.v > 2) return fail(NOOR_ERR_UNSUPPORTED, "JPEG: sampling factors");
+                comps.push_back(c);
+            }
+        } else if (m == 0xDA) {  // start of scan: the entropy-coded data follows the header
+            if (comps.empty()) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: scan before frame");
+            const int ns = b[q0];
+            if (ns != (int)comps.size()) return fail(NOOR_ERR_UNSUPPORTED, "JPEG: only interleaved single scans");
+            for (int i = 0; i < ns; ++i) {
+                const int id = b[q0 + 1 + 2 * i], t = b[q0 + 2 + 2 * i];
+                for (Comp& c : comps)
+                    if (c.id == id) {
+                        c.td = t >> 4 & 3;
+                        c.ta = t & 3;
+                    }
+            }
+            int hmax = 1, vmax = 1;
+            for (const Comp& c : comps) {
+                hmax = std::max(hmax, c.h);
+                vmax = std::max(vmax, c.v);
+            }
+            const int mcux = (w + 8 * hmax - 1) / (8 * hmax), mcuy = (h + 8 * vmax - 1) / (8 * vmax);
+            for (Comp& c : comps) {
+                if (!hd[c.td].ok || !ha[c.ta].ok) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: missing Huffman table");
+                c.bw = mcux * c.h * 8;
+                c.bh = mcuy * c.v * 8;
+                c.px.assign((size_t)c.bw * c.bh, 0);
+            }
+            JpegBits bs(b, q1);
+            std::vector<int> pred(comps.size(), 0);
+            int mcus = 0;
+            double blk[64], tmp[64];
+            for (int my = 0; my < mcuy; ++my)
+                for (int mx = 0; mx < mcux; ++mx) {
+                    if (restart && mcus && mcus % restart == 0) {
+                        bs.reset();
+                        std::fill(pred.begin(), pred.end(), 0);
+                    }
+                    ++mcus;
+                    for (size_t ci = 0; ci < comps.size(); ++ci) {
+                        Comp& c = comps[ci];
+                        for (int by = 0; by < c.v; ++by)
+                            for (int bx = 0; bx < c.h; ++bx) {
+                                int coef[64] = {0};
+                                const int t = jpeg_decode_huff(bs, hd[c.td]);
+                                if (t < 0 || t > 11) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad DC code");
+                                pred[ci] += jpeg_extend(bs.bits(t), t);
+                                coef[0] = pred[ci] * qt[c.tq][0];
+                                for (int k = 1; k < 64;) {
+                                    const int rs = jpeg_decode_huff(bs, ha[c.ta]);
+                                    if (rs < 0) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad AC code");
+                                    const int r = rs >> 4, sz = rs & 15;
+                                    if (sz == 0) {
+                                        if (r != 15) break;  // end of block
+                                        k += 16;
+                                        continue;
+                                    }
+                                    k += r;
+                                    if (k > 63) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: AC index");
+                                    coef[zz[k]] = jpeg_extend(bs.bits(sz), sz) * qt[c.tq][zz[k]];
+                                    ++k;
+                                }
+                                if (bs.starved > 1) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: truncated scan");
+                                // separable inverse DCT in binary64: f = 1/4 sum C(u) C(v) F cos cos
+                                for (int y = 0; y < 8; ++y)
+                                    for (int x = 0; x < 8; ++x) {
+                                        double acc = 0;
+                                        for (int u = 0; u < 8; ++u) acc += idct[x][u] * coef[y * 8 + u];
+                                        tmp[y * 8 + x] = acc / 2;
+                                    }
+                                for (int x = 0; x < 8; ++x)
+                                    for (int y = 0; y < 8; ++y) {
+                                        double acc = 0;
+                                        for (int v = 0; v < 8; ++v) acc += idct[y][v] * tmp[v * 8 + x];
+                                        blk[y * 8 + x] = acc / 2;
+                                    }
+                                const int ox = (mx * c.h + bx) * 8, oy = (my * c.v + by) * 8;
+                                for (int y = 0; y < 8; ++y)
+                                    for (int x = 0; x < 8; ++x) {
+                                        const long v = std::lround(blk[y * 8 + x] + 128.0);
+                                        c.px[(size_t)(oy + y) * c.bw + ox + x] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+                                    }
+                            }
+                    }
+                }
+            W = (uint32_t)w;
+            H = (uint32_t)h;
+            out.assign(4 * (size_t)W * H, 1.0f);
+            for (uint32_t y = 0; y < H; ++y)
+                for (uint32_t x = 0; x < W; ++x) {
+                    auto at = [&](const Comp& c) {
+                        const size_t sx = (size_t)x * c.h / hmax, sy = (size_t)y * c.v / vmax;
+                        return (double)c.px[sy * c.bw + sx];
+                    };
+                    float* o = &out[4 * ((size_t)y * W + x)];
+                    if (comps.size() == 1) {
+                        o[0] = o[1] = o[2] = std::pow((float)(at(comps[0]) / 255.0), 2.2f);
+                    } else {  // JFIF YCbCr -> RGB
+                        const double Y = at(comps[0]), cb = at(comps[1]) - 128.0, cr = at(comps[2]) - 128.0;
+                        const double rgb[3] = {Y + 1.402 * cr, Y - 0.344136 * cb - 0.714136 * cr, Y + 1.772 * cb};
+                        for (int k = 0; k < 3; ++k) {
+                            const double v = std::round(rgb[k]);
+                            o[k] = std::pow((float)((v < 0 ? 0 : (v > 255 ? 255 : v)) / 255.0), 2.2f);
+                        }
+                    }
+                }
+            return NOOR_OK;
+        }
+        p = q1;
+    }
+    return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: no image data");
+}
+
 float half_to_float(uint16_t h) {
     const uint32_t sign = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31, m = h & 1023;
     uint32_t bits;
@@ -970,6 +1223,7 @@ noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height,
     else if (b.size() >= 2 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) s = load_pfm(b, W, H, px);
     else if (b.size() >= 8 && std::memcmp(b.data(), "\x89PNG\r\n\x1a\n", 8) == 0) s = load_png(b, W, H, px);
     else if (b.size() >= 8 && b[0] == 0x76 && b[1] == 0x2f && b[2] == 0x31 && b[3] == 0x01) s = load_exr(b, W, H, px);
+    else if (b.size() >= 4 && b[0] == 0xFF && b[1] == 0xD8 && b[2] == 0xFF) s = load_jpeg(b, W, H, px);
     else if (b.size() >= 18 && ((b[2] >= 1 && b[2] <= 3) || (b[2] >= 9 && b[2] <= 11)) && b[1] <= 1 &&
              std::string(path).size() >= 4 &&
              (std::string(path).compare(std::string(path).size() - 4, 4, ".tga") == 0 ||

@@ -24,7 +24,8 @@ def _err(status, what):
 
 
 def load_image(path):
-    """(H, W, 4) float32, row 0 first: Radiance .hdr, PFM, binary PGM / PPM."""
+    """(H, W, 4) float32, row 0 first: Radiance .hdr, PFM, binary PGM / PPM,
+    PNG, TGA, OpenEXR, baseline JPEG (include/noor_loaders.h)."""
     w, h = C.c_uint32(), C.c_uint32()
     p = C.POINTER(C.c_float)()
     st = lib().noor_image_load(os.fsencode(path), C.byref(w), C.byref(h), C.byref(p))

@@ -495,6 +495,172 @@ def test_tga(tmp_path, kind, chans, rle, top):
     np.testing.assert_allclose(got[..., 3], img[..., 3] / 255.0 if chans == 4 else 1.0, rtol=1e-6)
 
 
+# JPEG: the Huffman tables of ITU T.81 Annex K.3 (luminance DC and AC)
+_DC_BITS = [0, 1, 5, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0]
+_DC_VALS = list(range(12))
+_AC_BITS = [0, 2, 1, 3, 3, 2, 4, 3, 5, 5, 4, 4, 0, 0, 1, 0x7d]
+_AC_VALS = [
+    0x01, 0x02, 0x03, 0x00, 0x04, 0x11, 0x05, 0x12, 0x21, 0x31, 0x41, 0x06, 0x13, 0x51, 0x61, 0x07,
+    0x22, 0x71, 0x14, 0x32, 0x81, 0x91, 0xa1, 0x08, 0x23, 0x42, 0xb1, 0xc1, 0x15, 0x52, 0xd1, 0xf0,
+    0x24, 0x33, 0x62, 0x72, 0x82, 0x09, 0x0a, 0x16, 0x17, 0x18, 0x19, 0x1a, 0x25, 0x26, 0x27, 0x28,
+    0x29, 0x2a, 0x34, 0x35, 0x36, 0x37, 0x38, 0x39, 0x3a, 0x43, 0x44, 0x45, 0x46, 0x47, 0x48, 0x49,
+    0x4a, 0x53, 0x54, 0x55, 0x56, 0x57, 0x58, 0x59, 0x5a, 0x63, 0x64, 0x65, 0x66, 0x67, 0x68, 0x69,
+    0x6a, 0x73, 0x74, 0x75, 0x76, 0x77, 0x78, 0x79, 0x7a, 0x83, 0x84, 0x85, 0x86, 0x87, 0x88, 0x89,
+    0x8a, 0x92, 0x93, 0x94, 0x95, 0x96, 0x97, 0x98, 0x99, 0x9a, 0xa2, 0xa3, 0xa4, 0xa5, 0xa6, 0xa7,
+    0xa8, 0xa9, 0xaa, 0xb2, 0xb3, 0xb4, 0xb5, 0xb6, 0xb7, 0xb8, 0xb9, 0xba, 0xc2, 0xc3, 0xc4, 0xc5,
+    0xc6, 0xc7, 0xc8, 0xc9, 0xca, 0xd2, 0xd3, 0xd4, 0xd5, 0xd6, 0xd7, 0xd8, 0xd9, 0xda, 0xe1, 0xe2,
+    0xe3, 0xe4, 0xe5, 0xe6, 0xe7, 0xe8, 0xe9, 0xea, 0xf1, 0xf2, 0xf3, 0xf4, 0xf5, 0xf6, 0xf7, 0xf8,
+    0xf9, 0xfa]
+_ZZ = [0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14,
+       21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53,
+       60, 61, 54, 47, 55, 62, 63]
+
+
+def _dct_matrix():
+    c = np.array([[(np.sqrt(0.5) if u == 0 else 1.0) * np.cos((2 * x + 1) * u * np.pi / 16) for x in range(8)]
+                  for u in range(8)]) / 2
+    return c  # F = c f c^T, f = c^T F c
+
+
+def _huff_codes(bits, vals):
+    codes, code, k = {}, 0, 0
+    for l in range(1, 17):
+        for _ in range(bits[l - 1]):
+            codes[vals[k]] = (code, l)
+            code += 1
+            k += 1
+        code <<= 1
+    return codes
+
+
+def _jpeg(path, comps, sampling, q=2, restart=0):
+    """A baseline JPEG written here: `comps` = planes of 8-bit samples (Y, or
+    Y Cb Cr with chroma at 1/sampling resolution), uniform quantiser q, the
+    Annex K luminance tables for every component.  Returns the dequantised
+    coefficient planes (what a decoder must invert)."""
+    C = _dct_matrix()
+    dc, ac = _huff_codes(_DC_BITS, _DC_VALS), _huff_codes(_AC_BITS, _AC_VALS)
+    H, W = comps[0].shape
+    hs = [sampling, 1, 1][:len(comps)]
+    mcu = 8 * sampling
+    mx, my = -(-W // mcu), -(-H // mcu)
+    out_bits, deq = [], [np.zeros((my * mcu // (sampling // h), mx * mcu // (sampling // h))) for h in hs]
+    def put(v, n):
+        for i in range(n - 1, -1, -1):
+            out_bits.append((v >> i) & 1)
+    def size(v):
+        return 0 if v == 0 else int(abs(v)).bit_length()
+    pred = [0] * len(comps)
+    data = bytearray()
+    def flush_bits():
+        while len(out_bits) % 8:
+            out_bits.append(1)
+        for i in range(0, len(out_bits), 8):
+            byte = int("".join(map(str, out_bits[i:i + 8])), 2)
+            data.append(byte)
+            if byte == 0xFF:
+                data.append(0)
+        out_bits.clear()
+    n_mcu = 0
+    for by in range(my):
+        for bx in range(mx):
+            if restart and n_mcu and n_mcu % restart == 0:
+                flush_bits()
+                data.extend([0xFF, 0xD0 + ((n_mcu // restart - 1) % 8)])
+                pred = [0] * len(comps)
+            n_mcu += 1
+            for ci, (plane, h) in enumerate(zip(comps, hs)):
+                for sy in range(h):
+                    for sx in range(h):
+                        y0, x0 = (by * h + sy) * 8, (bx * h + sx) * 8
+                        blk = np.zeros((8, 8))
+                        ph, pw = plane.shape
+                        for y in range(8):
+                            for x in range(8):
+                                blk[y, x] = plane[min(y0 + y, ph - 1), min(x0 + x, pw - 1)]
+                        F = np.rint(C @ (blk - 128.0) @ C.T / q).astype(int)
+                        deq[ci][y0:y0 + 8, x0:x0 + 8] = F * q
+                        zz = F.reshape(-1)[_ZZ]
+                        d = int(zz[0]) - pred[ci]
+                        pred[ci] = int(zz[0])
+                        code, l = dc[size(d)]
+                        put(code, l)
+                        put(d if d >= 0 else d + (1 << size(d)) - 1, size(d))
+                        run = 0
+                        for k in range(1, 64):
+                            v = int(zz[k])
+                            if v == 0:
+                                run += 1
+                                continue
+                            while run > 15:
+                                put(*ac[0xF0])
+                                run -= 16
+                            code, l = ac[(run << 4) | size(v)]
+                            put(code, l)
+                            put(v if v >= 0 else v + (1 << size(v)) - 1, size(v))
+                            run = 0
+                        if run:
+                            put(*ac[0x00])
+    flush_bits()
+    def seg(m, payload):
+        return bytes([0xFF, m]) + struct.pack(">H", len(payload) + 2) + payload
+    hdr = b"\xff\xd8" + seg(0xDB, bytes([0]) + bytes([q] * 64))
+    nc = len(comps)
+    sof = struct.pack(">BHHB", 8, H, W, nc) + b"".join(
+        bytes([i + 1, (h << 4) | h, 0]) for i, h in enumerate(hs))
+    hdr += seg(0xC0, sof)
+    hdr += seg(0xC4, bytes([0x00]) + bytes(_DC_BITS) + bytes(_DC_VALS))
+    hdr += seg(0xC4, bytes([0x10]) + bytes(_AC_BITS) + bytes(_AC_VALS))
+    if restart:
+        hdr += seg(0xDD, struct.pack(">H", restart))
+    hdr += seg(0xDA, bytes([nc]) + b"".join(bytes([i + 1, 0x00]) for i in range(nc)) + bytes([0, 63, 0]))
+    with open(path, "wb") as f:
+        f.write(hdr + bytes(data) + b"\xff\xd9")
+    return deq, C
+
+
+@pytest.mark.parametrize("kind,sampling,restart", [("grey", 1, 0), ("colour", 1, 0), ("colour", 2, 0),
+                                                   ("colour", 2, 3), ("grey", 1, 5)])
+def test_baseline_jpeg(tmp_path, kind, sampling, restart):
+    """Decoded texels against the inverse DCT of the coefficients written
+    (ITU T.81 A.3.3, level shift, JFIF YCbCr), within one 8-bit step, and
+    against the source image within the quantiser's error."""
+    rng = np.random.default_rng(sampling * 10 + restart)
+    H, W = 21, 30
+    yy, xx = np.mgrid[0:H, 0:W]
+    smooth = lambda ph: 128 + 90 * np.sin(xx / 5.0 + ph) * np.cos(yy / 7.0 - ph)  # noqa: E731
+    Y = np.clip(smooth(0.3) + rng.normal(0, 3, (H, W)), 0, 255).round()
+    if kind == "grey":
+        comps = [Y]
+    else:
+        cb = np.clip(smooth(1.1), 0, 255).round()[::sampling, ::sampling]
+        cr = np.clip(smooth(2.0), 0, 255).round()[::sampling, ::sampling]
+        comps = [Y, cb, cr]
+    deq, C = _jpeg(tmp_path / "x.jpg", comps, sampling, q=2, restart=restart)
+    got = load_image(str(tmp_path / "x.jpg"))
+    assert got.shape == (H, W, 4)
+    def plane(ci):
+        P = deq[ci]
+        out = np.zeros_like(P)
+        for y0 in range(0, P.shape[0], 8):
+            for x0 in range(0, P.shape[1], 8):
+                out[y0:y0 + 8, x0:x0 + 8] = C.T @ P[y0:y0 + 8, x0:x0 + 8] @ C
+        return np.clip(np.rint(out + 128.0), 0, 255)
+    Yd = plane(0)[:H, :W]
+    if kind == "grey":
+        want = np.repeat(Yd[..., None], 3, -1)
+    else:
+        s = sampling
+        cbd = plane(1).repeat(s, 0).repeat(s, 1)[:H, :W] - 128.0
+        crd = plane(2).repeat(s, 0).repeat(s, 1)[:H, :W] - 128.0
+        want = np.clip(np.rint(np.stack([Yd + 1.402 * crd, Yd - 0.344136 * cbd - 0.714136 * crd, Yd + 1.772 * cbd],
+                                        -1)), 0, 255)
+    v8 = got[..., :3] ** (1 / 2.2) * 255.0
+    assert np.max(np.abs(v8 - want)) <= 1.01
+    assert np.max(np.abs(v8[..., 0] - (Y if kind == "grey" else want[..., 0]))) <= 8.0
+    assert np.all(got[..., 3] == 1.0)
+
+
 def test_bad_files_fail_cleanly(tmp_path):
     _write(tmp_path / "bad.obj", "v 0 0 0\nf 1 2 3\n")
     with pytest.raises(abi.NoorError):
@@ -505,10 +671,22 @@ def test_bad_files_fail_cleanly(tmp_path):
         f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 4 +X 4\n\x01\x02")
     with pytest.raises(abi.NoorError):
         load_image(str(tmp_path / "t.hdr"))
-    with open(tmp_path / "x.jpg", "wb") as f:
-        f.write(b"\xff\xd8\xff\xe0")
+    with open(tmp_path / "x.gif", "wb") as f:
+        f.write(b"GIF89a")
     with pytest.raises(abi.NoorError) as e:
-        load_image(str(tmp_path / "x.jpg"))
+        load_image(str(tmp_path / "x.gif"))
+    assert e.value.status == abi.ERR_UNSUPPORTED
+    _jpeg(tmp_path / "t.jpg", [np.full((64, 64), 77.0)], 1)
+    data = open(tmp_path / "t.jpg", "rb").read()
+    with open(tmp_path / "t.jpg", "wb") as f:  # entropy-coded data cut short
+        f.write(data[:len(data) // 2])
+    with pytest.raises(abi.NoorError) as e:
+        load_image(str(tmp_path / "t.jpg"))
+    assert e.value.status == abi.ERR_INVALID_PAYLOAD
+    with open(tmp_path / "p.jpg", "wb") as f:  # progressive: refused
+        f.write(b"\xff\xd8\xff\xc2\x00\x0b\x08\x00\x10\x00\x10\x01\x01\x11\x00")
+    with pytest.raises(abi.NoorError) as e:
+        load_image(str(tmp_path / "p.jpg"))
     assert e.value.status == abi.ERR_UNSUPPORTED
     with open(tmp_path / "x.png", "wb") as f:
         f.write(b"\x89PNG\r\n\x1a\n\x00\x00")
