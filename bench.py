@@ -41,7 +41,7 @@ WIDE_NODE_BYTES = 56  # per wide-node visit: origin + steps (16 B), 8-bit child 
 # HBM bytes per dispatch of the dominant kernel on the default terrain workload,
 # from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB
 # calibration copy (commands and figures in profiles/pmc_traffic_terrain_1080p.md)
-MEASURED_HBM_BYTES = {("terrain", 1920, 1080, 32, "wave_extend"): 1167666429}
+MEASURED_HBM_BYTES = {("terrain", 1920, 1080, 32, "wave_extend"): 1154551032}
 
 
 def log(msg):

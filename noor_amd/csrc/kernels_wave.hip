@@ -74,7 +74,7 @@ struct WaveBuffers {
     uint32_t capacity = 0;
     int block = 64;
     int stack_lds = 0;
-    int steps = 16;            // traversal steps between refills of the persistent kernels
+    int steps = 32;            // traversal steps between refills of the persistent kernels
     uint32_t sb_items = 0;     // sample buffer entries
     uint32_t max_views = 0, max_segments = 0;
     // path state (one slot per path in flight)
