@@ -930,8 +930,10 @@ noor_status load_jpeg(const std::vector<unsigned char>& b, uint32_t& W, uint32_t
                 t.ok = true;
             }
         } else if (m == 0xDD) {
+            if (q0 + 2 > q1) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad DRI");
             restart = u16(q0);
         } else if (m == 0xC0 || m == 0xC1) {
+            if (q0 + 6 > q1 || !comps.empty()) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad frame header");
             if (b[q0] != 8) return fail(NOOR_ERR_UNSUPPORTED, "JPEG: only 8-bit samples");
             h = u16(q0 + 1);
             w = u16(q0 + 3);
@@ -950,6 +952,7 @@ noor_status load_jpeg(const std::vector<unsigned char>& b, uint32_t& W, uint32_t
             }
         } else if (m == 0xDA) {  // start of scan: the entropy-coded data follows the header
             if (comps.empty()) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: scan before frame");
+            if (q0 + 1 > q1 || q0 + 1 + 2 * (size_t)b[q0] + 3 > q1) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: bad scan header");
             const int ns = b[q0];
             if (ns != (int)comps.size()) return fail(NOOR_ERR_UNSUPPORTED, "JPEG: only interleaved single scans");
             for (int i = 0; i < ns; ++i) {

@@ -683,6 +683,12 @@ def test_bad_files_fail_cleanly(tmp_path):
     with pytest.raises(abi.NoorError) as e:
         load_image(str(tmp_path / "t.jpg"))
     assert e.value.status == abi.ERR_INVALID_PAYLOAD
+    for seg in (b"\xff\xc0\x00\x02", b"\xff\xdd\x00\x02", b"\xff\xda\x00\x02"):  # segments cut short
+        with open(tmp_path / "s.jpg", "wb") as f:
+            f.write(b"\xff\xd8" + seg)
+        with pytest.raises(abi.NoorError) as e:
+            load_image(str(tmp_path / "s.jpg"))
+        assert e.value.status == abi.ERR_INVALID_PAYLOAD
     with open(tmp_path / "p.jpg", "wb") as f:  # progressive: refused
         f.write(b"\xff\xd8\xff\xc2\x00\x0b\x08\x00\x10\x00\x10\x01\x01\x11\x00")
     with pytest.raises(abi.NoorError) as e:
