@@ -108,6 +108,22 @@ noor_status noor_camera_look_at(const float eye[3], const float look_at[3], cons
 noor_status noor_spec_fill(const noor_payload* p, uint16_t bounces, uint16_t rr,
                            int32_t skydome_type, uint8_t mis, noor_spec* out);
 
+/* Hosek-Wilkie RGB sky state for turbidity in [1, 10], ground albedo in
+ * [0, 1] and solar elevation in [0, pi/2] radians: the nine configuration
+ * parameters and the radiance scale per channel, blended from the published
+ * coefficient tables.  Replaces arhosek_rgb_skymodelstate_alloc_update
+ * (noor/external/ArHosekSkyModel.cpp:147-304,667-695); bit-identical cooks. */
+noor_status noor_hosek_cook_state(float turbidity, float albedo, float elevation, noor_hosek_state* out);
+
+/* The whole sky as HosekSky::update builds it (noor/header/hosek.h:61-83):
+ * one state per ground-albedo channel at elevation pi/2 - sun_lat, sun
+ * direction sphericalDirection(sun_lat, sun_lon) (y up), solar_scale (the
+ * reference's default 250) and solar_radius_factor x the terrestrial solar
+ * radius (default factor 2).  sun_lat in [0, pi/2].  Hand the result to
+ * noor_create or noor_update_hosek. */
+noor_status noor_hosek_cook(float turbidity, const float ground_albedo[3], float sun_lat, float sun_lon,
+                            float solar_scale, float solar_radius_factor, noor_hosek* out);
+
 #ifdef __cplusplus
 }
 #endif

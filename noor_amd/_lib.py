@@ -75,6 +75,9 @@ def lib():
         "noor_camera_look_at": (C.c_int, [P(C.c_float), P(C.c_float), P(C.c_float), C.c_float, C.c_float,
                                           C.c_float, C.c_uint32, C.c_uint32, C.c_int32, P(abi.Camera)]),
         "noor_spec_fill": (C.c_int, [C.c_void_p, C.c_uint16, C.c_uint16, C.c_int32, C.c_uint8, P(abi.Spec)]),
+        "noor_hosek_cook_state": (C.c_int, [C.c_float, C.c_float, C.c_float, P(abi.HosekState)]),
+        "noor_hosek_cook": (C.c_int, [C.c_float, P(C.c_float), C.c_float, C.c_float, C.c_float, C.c_float,
+                                      P(abi.Hosek)]),
         "noor_obj_load": (C.c_int, [C.c_char_p, C.c_int, C.c_float, P(C.c_void_p)]),
         "noor_obj_view_get": (C.c_int, [C.c_void_p, P(abi.ObjView)]),
         "noor_obj_free": (None, [C.c_void_p]),

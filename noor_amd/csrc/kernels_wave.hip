@@ -252,8 +252,10 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WaveBuffers B, int cur)
     if (t == 0) {
         ctl->item_next[nxt] = first + n_new;
         ctl->new_rays[cur] = S.bounces > 0 ? n_new : 0u;
-        // counters the previous iteration's kernels consumed
-        ctl->free_count[nxt] = 0;
+        // counters the previous iteration's kernels consumed; without bounces
+        // no path is traced, so the slots of the items started here go straight
+        // back to the next free queue (below) instead of through k_shade
+        ctl->free_count[nxt] = S.bounces > 0 ? 0u : n_new;
         ctl->ray_count[nxt] = 0;
         for (int c = 0; c < kClasses; ++c) ctl->class_count[c] = 0;
         ctl->shadow_count = 0;
@@ -272,9 +274,11 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WaveBuffers B, int cur)
     if (t >= n_new) return;
     const uint32_t item = first + t;
     if (S.bounces <= 0) {
-        // no bounce to trace: the sample is the (clamped) zero radiance
+        // no bounce to trace: the sample is the (clamped) zero radiance, and
+        // the slot stays free for the next item
         const f3 c = final_radiance(S, f3{0.0f, 0.0f, 0.0f});
         B.sb[item] = make_float4(c.x, c.y, c.z, 0.0f);
+        B.q_free[nxt][t] = p;
         return;
     }
     // segment of this item: segments are sorted by item_begin

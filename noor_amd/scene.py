@@ -382,6 +382,24 @@ def make_options(rng=abi.RNG_PCG32, tri_test=abi.TRI_WATERTIGHT, compat=abi.COMP
     return abi.Options(rng, tri_test, compat, stats)
 
 
+def cook_hosek(turbidity=3.5, ground_albedo=(1.0, 1.0, 1.0), sun_lat=np.pi / 4, sun_lon=np.pi / 4, scale=250.0,
+               solar_radius=2.0):
+    """The sky HosekSky::update cooks (noor/header/hosek.h:61-83), with the
+    reference's defaults: turbidity 3.5, white ground, sun at 45 degrees."""
+    hk = abi.Hosek()
+    alb = (C.c_float * 3)(*map(float, ground_albedo))
+    check(lib().noor_hosek_cook(float(turbidity), alb, float(sun_lat), float(sun_lon), float(scale),
+                                float(solar_radius), C.byref(hk)))
+    return hk
+
+
+def cook_hosek_state(turbidity, albedo, elevation):
+    """One channel-set state of arhosek_rgb_skymodelstate_alloc_update."""
+    st = abi.HosekState()
+    check(lib().noor_hosek_cook_state(float(turbidity), float(albedo), float(elevation), C.byref(st)))
+    return st
+
+
 def make_hosek(configs, radiances, solar_radius, turbidity=3.0, albedo=0.3, elevation=np.pi / 6,
                sun_lat=np.pi / 3, sun_lon=np.pi / 4, solar_scale=250.0, solar_radius_factor=2.0):
     """CudaHosekSky from a cooked Hosek-Wilkie state (HosekSky::update,

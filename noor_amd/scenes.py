@@ -14,15 +14,11 @@ There are no scene assets: every workload is generated here from a seed.
   adjoint_scene(...) the Cornell geometry without ceiling, 64 glossy materials
                     (albedo + roughness), lit by a 512x256 environment.
 """
-import json
-import os
 
 import numpy as np
 
 from . import abi
-from .scene import Mesh, SceneBuilder, make_camera, make_hosek, make_options, make_spec, material
-
-HERE = os.path.dirname(os.path.abspath(__file__))
+from .scene import Mesh, SceneBuilder, cook_hosek, make_camera, make_options, make_spec, material
 
 # Copper refractive index and absorption (RGB), as used by PBRT-v3 for "Cu".
 CU_ETA = (0.200438, 0.924033, 1.10221)
@@ -169,15 +165,12 @@ def random_rotation(rng):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def hosek_fixture():
-    with open(os.path.join(HERE, "data", "hosek_t3_a03_e30.json")) as fh:
-        return json.load(fh)
-
-
-def hosek_sky():
-    fx = hosek_fixture()
-    return make_hosek(fx["configs"], fx["radiances"], fx["solar_radius"], turbidity=3.0, albedo=0.3,
-                      elevation=np.radians(30.0), sun_lat=np.radians(60.0), sun_lon=np.pi / 4)
+def hosek_sky(turbidity=3.0, albedo=0.3, elevation_deg=30.0, sun_lon=np.pi / 4):
+    """The 1M-triangle scene's sky: Hosek-Wilkie, turbidity 3, grey ground of
+    albedo 0.3, sun 30 degrees above the horizon, cooked as HosekSky::update
+    does (noor/header/hosek.h:61-83)."""
+    lat = np.float32(np.radians(90.0 - elevation_deg))
+    return cook_hosek(turbidity, (albedo, albedo, albedo), lat, np.float32(sun_lon))
 
 
 def env_map(width, height, seed, lobes=3, sigma_deg=2.0, peak=1.0e4):

@@ -46,6 +46,8 @@ def lib():
     L.oracle_scene_dist_func.argtypes = [vp, P(C.c_int), P(C.c_int)]
     L.oracle_scene_dist_cdf.restype = P(C.c_float)
     L.oracle_scene_dist_cdf.argtypes = [vp]
+    L.oracle_scene_sky_bake.restype = P(C.c_float)
+    L.oracle_scene_sky_bake.argtypes = [vp, P(C.c_int), P(C.c_int)]
     for sfx, real in (("f32", C.c_float), ("f64", C.c_double)):
         f = getattr(L, f"oracle_render_{sfx}")
         f.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P(real),
@@ -208,6 +210,15 @@ class OracleScene:
         func = np.ctypeslib.as_array(fp, shape=(n,)).copy()
         cdf = np.ctypeslib.as_array(lib().oracle_scene_dist_cdf(self._h), shape=(n,)).copy()
         return w.value, h.value, func, cdf
+
+    def sky_bake(self):
+        """The baked physical sky (H, W, 4): sin(theta) * sky at texel corners."""
+        w = C.c_int()
+        h = C.c_int()
+        p = lib().oracle_scene_sky_bake(self._h, C.byref(w), C.byref(h))
+        if not p:
+            return None
+        return np.ctypeslib.as_array(p, shape=(h.value, w.value, 4)).copy()
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -121,6 +121,7 @@ int oracle_dist_build(const float* rgba, int tw, int th, orc_dist2d* d);
 void oracle_dist_free(orc_dist2d* d);
 const float* oracle_scene_dist_func(const orc_scene* s, int* w, int* h);
 const float* oracle_scene_dist_cdf(const orc_scene* s);
+const float* oracle_scene_sky_bake(const orc_scene* s, int* w, int* h);
 
 /* shared float helpers (oracle_scene.c) */
 float oracle_tex_bilinear(const float* rgba, int w, int h, float u, float v, int c);

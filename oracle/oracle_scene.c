@@ -238,6 +238,14 @@ const float* oracle_scene_dist_func(const orc_scene* s, int* w, int* h) {
 }
 const float* oracle_scene_dist_cdf(const orc_scene* s) { return s->dist.cdf; }
 
+/* The physical sky's baked texture (sin(theta) * sky, RGBA), or NULL when the
+   scene's sky is not PHYSICAL. */
+const float* oracle_scene_sky_bake(const orc_scene* s, int* w, int* h) {
+    if (w) *w = (int)s->sky_w;
+    if (h) *h = (int)s->sky_h;
+    return s->sky_bake;
+}
+
 /* ---- scene ----------------------------------------------------------------- */
 
 static void tex_rgb(const noor_payload_view* p, uint32_t idx, float out[3]) {
