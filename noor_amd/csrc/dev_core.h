@@ -925,6 +925,27 @@ __device__ __forceinline__ f3 tex_bilinear(const float4* tex, int w, int h, floa
     return r;
 }
 
+// The fourth channel of tex_bilinear's lookup.
+__device__ __forceinline__ float tex_bilinear_w(const float4* tex, int w, int h, float u, float v) {
+    if (u != u) u = 0.0f;
+    if (v != v) v = 0.0f;
+    float x = u * (float)w - 0.5f;
+    float y = v * (float)h - 0.5f;
+    x = clampr(x, -2.0f, (float)w + 1.0f);
+    y = clampr(y, -2.0f, (float)h + 1.0f);
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int ix = (int)fx0, iy = (int)fy0;
+    const int i0 = ix < 0 ? 0 : (ix > w - 1 ? w - 1 : ix);
+    const int i1 = ix + 1 < 0 ? 0 : (ix + 1 > w - 1 ? w - 1 : ix + 1);
+    const int j0 = iy < 0 ? 0 : (iy > h - 1 ? h - 1 : iy);
+    const int j1 = iy + 1 < 0 ? 0 : (iy + 1 > h - 1 ? h - 1 : iy + 1);
+    const float t00 = ldg(tex + (size_t)j0 * w + i0).w, t10 = ldg(tex + (size_t)j0 * w + i1).w;
+    const float t01 = ldg(tex + (size_t)j1 * w + i0).w, t11 = ldg(tex + (size_t)j1 * w + i1).w;
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    return by * (bx * t00 + ax * t10) + ay * (bx * t01 + ax * t11);
+}
+
 // ---- material image textures (noor/cuda/texture.cuh:30-423) -------------------------------
 // Software filtering restated identically in the oracle: bilinear within a
 // level (texel centres at half-integers, wrap addressing) and linear between
@@ -2427,7 +2448,7 @@ struct Tape {
 // differentials of `ray` and receives those of the next ray.
 template <bool TEX = false>
 __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, const Ray& ray, bool hit, const Hit& h, Nee& nee,
-                             f3* lookat, Tape* tp = nullptr, RayDiff* rd = nullptr) {
+                             float4* lookat, Tape* tp = nullptr, RayDiff* rd = nullptr) {
     const int rk = S.rng_kind;
     nee.has = 0;
     nee.na = 0;
@@ -2451,7 +2472,8 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
     Isect I;
     HitDiff hd;
     make_isect<TEX>(S, ray, h, I, rd, &hd);
-    if (ps.bounce == 0 && lookat) *lookat = I.p;
+    // autofocus point: the first hit of the sample, w = 1 (noor/cuda/path_tracer.cu:40-42)
+    if (ps.bounce == 0 && lookat) *lookat = make_float4(I.p.x, I.p.y, I.p.z, 1.0f);
     // the material with its image textures looked up (TEX), else the constants
     DevMat mt;
     if (TEX) {

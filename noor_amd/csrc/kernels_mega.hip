@@ -61,10 +61,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         if (x >= W || y >= H) continue;
         const uint32_t gid = y * W + x;
         float4 acc = DIRECT ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : film[gid];
-        f3 lookat = f3{0.0f, 0.0f, 0.0f};
+        // the autofocus point of the centre pixel's last sample: its first hit,
+        // or zero when that sample's camera ray escapes
+        float4 lookat = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const bool center = (gid == S.cam.center);
         for (uint32_t s = 0; s < spp; ++s) {
             const uint32_t frame = frame0 + s;
+            lookat = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             PathState ps;
             ps.L = f3{0.0f, 0.0f, 0.0f};
             ps.beta = f3{1.0f, 1.0f, 1.0f};
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         }
         if (DIRECT) direct[gid] = acc;
         else film[gid] = acc;
-        if (center && lookat_out) *lookat_out = make_float4(lookat.x, lookat.y, lookat.z, 1.0f);
+        if (center && lookat_out) *lookat_out = lookat;
     }
     flush_stats(stats, ext, shd, cnt, cns, overflow);
 }
@@ -197,26 +200,29 @@ hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, in
     return hipGetLastError();
 }
 
-// debug_skydome (noor/cuda/path_tracer.cu:102-122): paints the environment
-// over the film; importance-sample dots when MIS is on.
-__global__ void k_debug_skydome(const DevScene* __restrict__ Sp, float4* film, uint32_t frame) {
+// debug_skydome (noor/cuda/path_tracer.cu:102-122): paints the sky texture
+// (the sin(theta)-weighted bake of a physical sky, or the HDR environment)
+// over the film, read bilinearly at u = (x / w, y / h) with its four channels
+// (noor/cuda/skydome.cuh:87-89); importance-sample dots when MIS is on.
+// Pixels hit by a dot are written by two threads in no set order, as in the
+// reference.
+__global__ void k_debug_skydome(const DevScene* __restrict__ Sp, float4* film, uint32_t frame, const float4* tex,
+                                int tw, int th) {
     const DevScene& S = *Sp;
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
     const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= S.cam.w || y >= S.cam.h) return;
     const uint32_t id = y * S.cam.w + x;
     const float u = x / (float)S.cam.w, v = y / (float)S.cam.h;
-    f3 c;
-    if (S.skydome_type == NOOR_SKY_PHYSICAL) {
-        const float phi = u * K_2PI, theta = v * K_PI;
-        const float sT = nm_sinf(theta), cT = nm_cosf(theta);
-        c = hosek_query(*S.hosek, nrm3(f3{sT * nm_cosf(phi), cT, sT * nm_sinf(phi)}));
-    } else if (S.skydome_type == NOOR_SKY_CONSTANT) {
-        c = f3{0.25f, 0.25f, 0.25f};
-    } else {
-        c = tex_bilinear(S.env, (int)S.env_w, (int)S.env_h, u, v);
+    float4 c = make_float4(0.25f, 0.25f, 0.25f, 1.0f);
+    if (S.skydome_type != NOOR_SKY_CONSTANT) {
+        c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (tex) {
+            const f3 rgb = tex_bilinear(tex, tw, th, u, v);
+            c = make_float4(rgb.x, rgb.y, rgb.z, tex_bilinear_w(tex, tw, th, u, v));
+        }
     }
-    film[id] = make_float4(c.x, c.y, c.z, 1.0f);
+    film[id] = c;
     if (S.mis) {
         Rng rng;
         rng_init(rng, S.rng_kind, id, frame);
@@ -233,8 +239,11 @@ __global__ void k_debug_skydome(const DevScene* __restrict__ Sp, float4* film, u
 hipError_t launch_debug_skydome(const Context& c, uint32_t frame) {
     hipError_t se = sync_scene(c);
     if (se != hipSuccess) return se;
+    const bool phys = c.S.skydome_type == NOOR_SKY_PHYSICAL;
+    const float4* tex = phys ? c.sky_bake : c.S.env;
+    const int tw = phys ? 2048 : (int)c.S.env_w, th = phys ? 2048 : (int)c.S.env_h;
     hipLaunchKernelGGL(k_debug_skydome, dim3((c.S.cam.w + 15) / 16, (c.S.cam.h + 15) / 16), dim3(256), 0, c.stream,
-                       c.d_scene, c.film, frame);
+                       c.d_scene, c.film, frame, tex, tw, th);
     return hipGetLastError();
 }
 

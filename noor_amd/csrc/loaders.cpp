@@ -15,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <new>
 #include <sstream>
 #include <string>
 #include <tuple>
@@ -195,7 +196,7 @@ extern "C" {
 
 const char* noor_loader_last_error(void) { return g_err.c_str(); }
 
-noor_status noor_obj_load(const char* path, int unitize, float crease_deg, noor_obj** out) {
+static noor_status obj_load(const char* path, int unitize, float crease_deg, noor_obj** out) {
     if (!path || !out) return NOOR_ERR_INVALID_ARG;
     *out = nullptr;
     std::ifstream f(path);
@@ -457,6 +458,11 @@ void rgbe_to_float(const unsigned char* p, float* o) {
     o[3] = 1.0f;
 }
 
+// Images beyond 2^28 pixels (4 GiB of RGBA32F) are refused from the header,
+// before anything is allocated.
+constexpr uint64_t kMaxPixels = 1ull << 28;
+bool too_large(uint64_t w, uint64_t h) { return w * h > kMaxPixels; }
+
 noor_status load_hdr(const std::vector<unsigned char>& b, uint32_t& W, uint32_t& H, std::vector<float>& out) {
     size_t p = 0;
     auto getline_ = [&](std::string& s) {
@@ -476,6 +482,7 @@ noor_status load_hdr(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     if (std::sscanf(s.c_str(), "%3s %d %3s %d", ya, &h, xa, &w) != 4 || std::strcmp(ya, "-Y") != 0 ||
         std::strcmp(xa, "+X") != 0 || w <= 0 || h <= 0 || w > 1 << 16 || h > 1 << 16)
         return fail(NOOR_ERR_UNSUPPORTED, "HDR: only -Y H +X W orientation");
+    if (too_large(w, h)) return fail(NOOR_ERR_INVALID_PAYLOAD, "HDR: image too large");
     W = (uint32_t)w;
     H = (uint32_t)h;
     out.assign(4 * (size_t)W * H, 0.0f);
@@ -539,6 +546,7 @@ noor_status load_pnm(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     if (!pnm_token(b, p, w) || !pnm_token(b, p, h) || !pnm_token(b, p, mx) || w <= 0 || h <= 0 || mx <= 0 ||
         mx > 65535 || w > 1 << 16 || h > 1 << 16)
         return fail(NOOR_ERR_INVALID_PAYLOAD, "PNM: bad header");
+    if (too_large(w, h)) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNM: image too large");
     ++p;  // the single whitespace byte after maxval
     const int bytes = mx > 255 ? 2 : 1;
     if (p + (size_t)w * h * ch * bytes > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNM: truncated");
@@ -562,6 +570,7 @@ noor_status load_pfm(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     long w, h;
     if (!pnm_token(b, p, w) || !pnm_token(b, p, h) || w <= 0 || h <= 0 || w > 1 << 16 || h > 1 << 16)
         return fail(NOOR_ERR_INVALID_PAYLOAD, "PFM: bad header");
+    if (too_large(w, h)) return fail(NOOR_ERR_INVALID_PAYLOAD, "PFM: image too large");
     while (p < b.size() && std::isspace(b[p])) ++p;
     std::string sc;
     while (p < b.size() && !std::isspace(b[p])) sc += (char)b[p++];
@@ -630,6 +639,7 @@ noor_status load_png(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
                           ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
     if (!chans || !depth_ok || w == 0 || h == 0 || w > 1 << 16 || h > 1 << 16)
         return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad header");
+    if (too_large(w, h)) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: image too large");
     if (interlace > 1) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: bad interlace method");
     if (ctype == 3 && (plte.empty() || plte.size() % 3)) return fail(NOOR_ERR_INVALID_PAYLOAD, "PNG: no palette");
     const size_t bits = (size_t)chans * depth, bpp = std::max<size_t>(1, bits / 8);
@@ -735,6 +745,7 @@ noor_status load_tga(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     const int base = rle ? itype - 8 : itype;
     if (!(base == 1 || base == 2 || base == 3) || w == 0 || h == 0)
         return fail(NOOR_ERR_UNSUPPORTED, "TGA: only colour-mapped, true-colour and grey images");
+    if (too_large(w, h)) return fail(NOOR_ERR_INVALID_PAYLOAD, "TGA: image too large");
     const bool mapped = base == 1;
     if ((mapped && (cmtype != 1 || bits != 8 || (cmbits != 24 && cmbits != 32))) ||
         (base == 2 && bits != 24 && bits != 32) || (base == 3 && bits != 8))
@@ -940,7 +951,7 @@ noor_status load_jpeg(const std::vector<unsigned char>& b, uint32_t& W, uint32_t
             const int nc = b[q0 + 5];
             if ((nc != 1 && nc != 3) || q0 + 6 + 3 * (size_t)nc > q1 || w == 0 || h == 0)
                 return fail(NOOR_ERR_UNSUPPORTED, "JPEG: only grey or three-component images");
-            if ((uint64_t)w * h > (1ull << 28)) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: image too large");
+            if (too_large(w, h)) return fail(NOOR_ERR_INVALID_PAYLOAD, "JPEG: image too large");
             for (int i = 0; i < nc; ++i) {
                 Comp c;
                 c.id = b[q0 + 6 + 3 * i];
@@ -963,6 +974,9 @@ noor_status load_jpeg(const std::vector<unsigned char>& b, uint32_t& W, uint32_t
                         c.ta = t & 3;
                     }
             }
+            // a scan of one component is not interleaved: one 8x8 block per MCU in
+            // raster order whatever its sampling factors say (T.81 A.2.2)
+            if (comps.size() == 1) comps[0].h = comps[0].v = 1;
             int hmax = 1, vmax = 1;
             for (const Comp& c : comps) {
                 hmax = std::max(hmax, c.h);
@@ -1128,6 +1142,7 @@ noor_status load_exr(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     const int64_t w = (int64_t)win[2] - win[0] + 1, h = (int64_t)win[3] - win[1] + 1;
     if (chans.empty() || w <= 0 || h <= 0 || w > 1 << 16 || h > 1 << 16)
         return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad header");
+    if (too_large(w, h)) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: image too large");
     if (comp < 0 || comp > 3) return fail(NOOR_ERR_UNSUPPORTED, "EXR: compression other than NONE, RLE, ZIPS, ZIP");
     size_t line_bytes = 0;
     for (const Ch& c : chans) {
@@ -1145,9 +1160,10 @@ noor_status load_exr(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
     for (size_t k = 0; k < nchunks; ++k) {
         uint64_t off;
         std::memcpy(&off, &b[p + 8 * k], 8);
-        if (off + 8 > b.size()) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad chunk offset");
+        // compared as sizes so that an offset near 2^64 cannot wrap
+        if (off > b.size() || b.size() - off < 8) return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad chunk offset");
         const int32_t y0 = rd32((size_t)off) - win[1], size = rd32((size_t)off + 4);
-        if (y0 < 0 || y0 >= h || size < 0 || off + 8 + (uint64_t)size > b.size())
+        if (y0 < 0 || y0 >= h || size < 0 || (uint64_t)size > b.size() - off - 8)
             return fail(NOOR_ERR_INVALID_PAYLOAD, "EXR: bad chunk");
         const int ny = (int)std::min<int64_t>(lines, h - y0);
         const size_t want = line_bytes * ny;
@@ -1213,7 +1229,7 @@ noor_status load_exr(const std::vector<unsigned char>& b, uint32_t& W, uint32_t&
 
 }  // namespace
 
-noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba) {
+static noor_status image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba) {
     if (!path || !width || !height || !rgba) return NOOR_ERR_INVALID_ARG;
     *rgba = nullptr;
     std::vector<unsigned char> b;
@@ -1240,6 +1256,28 @@ noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height,
     *height = H;
     *rgba = o;
     return NOOR_OK;
+}
+
+// The C entry points: no C++ exception crosses them (an allocation that
+// fails becomes NOOR_ERR_OUT_OF_MEMORY).
+noor_status noor_obj_load(const char* path, int unitize, float crease_deg, noor_obj** out) {
+    try {
+        return obj_load(path, unitize, crease_deg, out);
+    } catch (const std::bad_alloc&) {
+        return fail(NOOR_ERR_OUT_OF_MEMORY, "OBJ: out of memory");
+    } catch (...) {
+        return fail(NOOR_ERR_INVALID_PAYLOAD, "OBJ: unreadable file");
+    }
+}
+
+noor_status noor_image_load(const char* path, uint32_t* width, uint32_t* height, float** rgba) {
+    try {
+        return image_load(path, width, height, rgba);
+    } catch (const std::bad_alloc&) {
+        return fail(NOOR_ERR_OUT_OF_MEMORY, "image: out of memory");
+    } catch (...) {
+        return fail(NOOR_ERR_INVALID_PAYLOAD, "image: unreadable file");
+    }
 }
 
 void noor_image_free(float* rgba) { delete[] rgba; }

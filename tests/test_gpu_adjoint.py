@@ -1,12 +1,18 @@
-"""GPU: the path-replay adjoint against FP64 central differences.
+"""GPU: the path-replay adjoint against FP64 central differences (C4).
 
-The loss is L2 between a one-sample render at the start parameters (all
-0.5) and the one-sample render at the seeded ground truth.  Finite
-differences use the FP64 oracle with detached sampling: the random decisions
-come from a frozen copy of the start parameters and only the evaluation
-parameters move, so the FD and the adjoint differentiate the same smooth
-per-seed function.  Tolerance: 1e-3 relative, against max(|FD|, 1% of the
-largest |FD| of the parameter group)."""
+C4 at its stated size: the Cornell geometry (6x subdivided sphere), 512 x 512,
+one sample per pixel, depth 4, 64 materials (albedo RGB + GGX roughness) and a
+512 x 256 environment.  The loss is L2 between a one-sample render at the
+start parameters (all 0.5) and the one-sample render at the seeded ground
+truth.  Finite differences (h = 1e-3) use the FP64 oracle with detached
+sampling: the random decisions come from a frozen copy of the start
+parameters and only the evaluation parameters move, so the FD and the adjoint
+differentiate the same smooth per-seed function.  256 parameters are drawn
+uniformly within their groups (all 64 roughnesses, 64 of the 192 albedo
+channels, 128 of the 393,216 environment channels); each must agree within
+1e-3 relative.  The render is linear in each environment channel and a
+polynomial of low degree in each albedo, so central differences are exact up
+to rounding there; roughness enters through the GGX terms (error O(h^2))."""
 import numpy as np
 import pytest
 import torch
@@ -24,7 +30,9 @@ TOL = 1e-3
 
 @pytest.fixture(scope="module")
 def setup():
-    wl = scenes.adjoint_scene(subdiv=3, width=48, height=48)
+    wl = scenes.adjoint_scene()
+    assert (wl.camera.w, wl.camera.h, wl.spec.bounces) == (512, 512, 4)
+    assert wl.info["n_materials"] == 64 and wl.info["env"].shape[:2] == (256, 512)
     M = wl.info["n_materials"]
     env_gt = wl.info["env"]
     eh, ew = env_gt.shape[:2]
@@ -54,6 +62,7 @@ def _oracle_scenes(wl, p0):
     return out, env
 
 
+@pytest.mark.timeout(900)
 def test_adjoint_matches_fp64_central_differences(setup):
     wl, r, target, p0 = setup
     a = p0["albedo"].clone().requires_grad_(True)
@@ -73,18 +82,15 @@ def test_adjoint_matches_fp64_central_differences(setup):
     base = loss64()
     print(f"loss gpu {loss.item():.6f} oracle fp64 {base:.6f}")
     assert base == pytest.approx(loss.item(), rel=1e-4)
-    # parameters: the largest gradients of each group plus a few others
+    # parameters drawn uniformly within each group
     rng = np.random.default_rng(0)
     M = ga.shape[0]
-    picks = []
-    for m in list(np.argsort(-np.abs(gr))[:6]) + list(rng.choice(M, 2, replace=False)):
-        picks.append(("rough", int(m), 0))
-    flat = np.abs(ga).ravel()
-    for k in list(np.argsort(-flat)[:6]) + list(rng.choice(flat.size, 2, replace=False)):
+    picks = [("rough", m, 0) for m in range(M)]
+    for k in rng.choice(ga.size, 64, replace=False):
         picks.append(("albedo", int(k // 3), int(k % 3)))
-    flat = np.abs(ge).ravel()
-    for k in np.argsort(-flat)[:10]:
+    for k in rng.choice(ge.size, 128, replace=False):
         picks.append(("env", int(k // 3), int(k % 3)))
+    assert len(picks) == 256
     results = {"rough": [], "albedo": [], "env": []}
     for kind, i, c in picks:
         vals = []
@@ -120,9 +126,10 @@ def test_adjoint_matches_fp64_central_differences(setup):
     for kind, rows in results.items():
         g = np.array([x for x, _ in rows])
         fd = np.array([y for _, y in rows])
-        floor = 1e-2 * np.abs(fd).max()
-        rel = np.abs(g - fd) / np.maximum(np.abs(fd), floor)
-        print(f"{kind}: max rel err {rel.max():.2e}")
+        zero = (g == 0.0) & (fd == 0.0)  # parameters no path reads
+        rel = np.where(zero, 0.0, np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300))
+        print(f"{kind}: {len(rows)} parameters, {zero.sum()} unread, max rel err {rel.max():.2e}, "
+              f"median {np.median(rel[~zero]) if (~zero).any() else 0:.2e}")
         assert rel.max() <= TOL, (kind, rel)
 
 

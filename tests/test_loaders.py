@@ -533,7 +533,7 @@ def _huff_codes(bits, vals):
     return codes
 
 
-def _jpeg(path, comps, sampling, q=2, restart=0):
+def _jpeg(path, comps, sampling, q=2, restart=0, grey_factors=None):
     """A baseline JPEG written here: `comps` = planes of 8-bit samples (Y, or
     Y Cb Cr with chroma at 1/sampling resolution), uniform quantiser q, the
     Annex K luminance tables for every component.  Returns the dequantised
@@ -608,6 +608,9 @@ def _jpeg(path, comps, sampling, q=2, restart=0):
     nc = len(comps)
     sof = struct.pack(">BHHB", 8, H, W, nc) + b"".join(
         bytes([i + 1, (h << 4) | h, 0]) for i, h in enumerate(hs))
+    if grey_factors is not None:  # a one-component frame stating other factors (the scan is unchanged)
+        assert nc == 1
+        sof = sof[:-2] + bytes([grey_factors, 0])
     hdr += seg(0xC0, sof)
     hdr += seg(0xC4, bytes([0x00]) + bytes(_DC_BITS) + bytes(_DC_VALS))
     hdr += seg(0xC4, bytes([0x10]) + bytes(_AC_BITS) + bytes(_AC_VALS))
@@ -661,6 +664,52 @@ def test_baseline_jpeg(tmp_path, kind, sampling, restart):
     assert np.all(got[..., 3] == 1.0)
 
 
+@pytest.mark.parametrize("factors", [0x22, 0x21, 0x12])
+def test_grey_jpeg_ignores_sampling_factors(tmp_path, factors):
+    """A one-component scan is not interleaved: one 8x8 block per MCU in
+    raster order whatever the frame's sampling factors (T.81 A.2.2)."""
+    rng = np.random.default_rng(factors)
+    Y = np.clip(120 + 60 * rng.standard_normal((27, 35)), 0, 255).round()
+    _jpeg(tmp_path / "a.jpg", [Y], 1)
+    _jpeg(tmp_path / "b.jpg", [Y], 1, grey_factors=factors)
+    np.testing.assert_array_equal(load_image(str(tmp_path / "b.jpg")), load_image(str(tmp_path / "a.jpg")))
+
+
+def test_exr_chunk_offset_near_2_64(tmp_path):
+    """An offset table entry close to 2^64 must not wrap past the size check."""
+    img = np.ones((4, 5, 3), np.float32)
+    _exr(tmp_path / "ok.exr", img, 0, True)
+    data = bytearray(open(tmp_path / "ok.exr", "rb").read())
+    load_image(str(tmp_path / "ok.exr"))
+    # the offset table follows the header (4 chunks of one line, no compression)
+    first = struct.unpack_from("<Q", data, len(data) - 4 * (8 + 5 * 3 * 2) - 4 * 8)[0]
+    assert first == len(data) - 4 * (8 + 5 * 3 * 2)
+    struct.pack_into("<Q", data, len(data) - 4 * (8 + 5 * 3 * 2) - 4 * 8, 0xFFFFFFFFFFFFFFF8)
+    with open(tmp_path / "bad.exr", "wb") as f:
+        f.write(bytes(data))
+    with pytest.raises(abi.NoorError) as e:
+        load_image(str(tmp_path / "bad.exr"))
+    assert e.value.status == abi.ERR_INVALID_PAYLOAD and "chunk offset" in str(e.value)
+
+
+def test_header_sizes_are_capped(tmp_path):
+    """Headers stating more than 2^28 pixels are refused before any allocation."""
+    cases = {
+        "big.hdr": b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 65536 +X 65536\n",
+        "big.pfm": b"PF\n65536 65536\n-1.0\n",
+        "big.ppm": b"P6\n65536 65536\n255\n",
+        "big.tga": bytes([0, 0, 2]) + bytes(9) + struct.pack("<HH", 65535, 65535) + bytes([24, 0]),
+        "big.png": b"\x89PNG\r\n\x1a\n" + struct.pack(">I", 13) + b"IHDR" +
+                   struct.pack(">IIBBBBB", 65536, 65536, 8, 2, 0, 0, 0) + b"\0\0\0\0",
+    }
+    for name, data in cases.items():
+        with open(tmp_path / name, "wb") as f:
+            f.write(data)
+        with pytest.raises(abi.NoorError) as e:
+            load_image(str(tmp_path / name))
+        assert e.value.status == abi.ERR_INVALID_PAYLOAD and "too large" in str(e.value), name
+
+
 def test_bad_files_fail_cleanly(tmp_path):
     _write(tmp_path / "bad.obj", "v 0 0 0\nf 1 2 3\n")
     with pytest.raises(abi.NoorError):
@@ -689,6 +738,14 @@ def test_bad_files_fail_cleanly(tmp_path):
         with pytest.raises(abi.NoorError) as e:
             load_image(str(tmp_path / "s.jpg"))
         assert e.value.status == abi.ERR_INVALID_PAYLOAD
+    # a valid frame header, then scan headers shorter than they claim
+    sof = b"\xff\xc0\x00\x0b\x08\x00\x10\x00\x10\x01\x01\x11\x00"
+    for sos in (b"\xff\xda\x00\x02", b"\xff\xda\x00\x04\x03\x01"):  # empty; Ns = 3 in 2 bytes
+        with open(tmp_path / "q.jpg", "wb") as f:
+            f.write(b"\xff\xd8" + sof + sos)
+        with pytest.raises(abi.NoorError) as e:
+            load_image(str(tmp_path / "q.jpg"))
+        assert e.value.status == abi.ERR_INVALID_PAYLOAD and "bad scan header" in str(e.value)
     with open(tmp_path / "p.jpg", "wb") as f:  # progressive: refused
         f.write(b"\xff\xd8\xff\xc2\x00\x0b\x08\x00\x10\x00\x10\x01\x01\x11\x00")
     with pytest.raises(abi.NoorError) as e:
