@@ -87,6 +87,12 @@ struct Context {
     std::vector<void*> allocs;
     float4* film = nullptr;
     uint32_t film_w = 0, film_h = 0;
+    // the rounding errors of the last noor_render_sample (one float4 per
+    // pixel), and which sample they belong to (its output buffer and stream)
+    float4* sample_lo = nullptr;
+    size_t sample_lo_px = 0;
+    const float* sample_lo_of = nullptr;
+    uint32_t sample_lo_stream = 0;
     float4* sky_bake = nullptr;
     float* dfunc = nullptr;
     float* dcdf = nullptr;
@@ -126,7 +132,10 @@ inline hipError_t sync_scene(const Context& cc) {
 
 // kernels_mega.hip
 // `direct`: write each pixel's last sample there instead of updating the film
-hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct = nullptr);
+// `direct_lo` (DIRECT only, may be null): the rounding error of each sample's
+// radiance sum, which the adjoint uses to recover the radiance after a bounce
+hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct = nullptr,
+                             float4* direct_lo = nullptr);
 hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits);
 hipError_t launch_debug_skydome(const Context& c, uint32_t frame);
 // kernels_env.hip
@@ -135,8 +144,9 @@ hipError_t launch_dist_build(const Context& c, const float4* tex, int tw, int th
                              int H);
 // kernels_adjoint.hip
 hipError_t launch_set_params(const Context& c, const float* albedo, const float* rough, uint32_t n, const float* env_rgb);
-hipError_t launch_adjoint(const Context& c, uint32_t stream, const float* radiance, const float* grad, float* g_albedo,
-                          float* g_rough, uint32_t num_mats, float* g_env);
+// `radiance_lo`: the forward's direct_lo for the same sample, or null
+hipError_t launch_adjoint(const Context& c, uint32_t stream, const float* radiance, const float4* radiance_lo,
+                          const float* grad, float* g_albedo, float* g_rough, uint32_t num_mats, float* g_env);
 // kernels_wave.hip
 hipError_t wave_render(Context& c, uint32_t frame0, uint32_t spp);
 hipError_t wave_render_batch(Context& c, const noor_view* views, uint32_t n, float4* films, const size_t* offsets);

@@ -2295,6 +2295,7 @@ __device__ __forceinline__ Ray gen_ray(const noor_camera& cam, uint32_t x, uint3
 // beta_prev * (visible_a + visible_b) to L exactly as `L += beta * direct`.
 struct PathState {
     f3 L, beta;
+    f3 Lc;  // rounding error of the additions into L (compensated paths only)
     Rng rng;
     f3 o, d;
     uint32_t bounce;
@@ -2308,11 +2309,37 @@ struct Nee {
     int has;           // 1 when `L += beta * Ld` runs for this bounce
 };
 
+// s = a + b, with the rounding error of the addition added to *err
+// (Knuth's TwoSum: exact in binary32 without contraction).
+__device__ __forceinline__ float two_sum(float a, float b, float& err) {
+    const float s = a + b;
+    const float bb = s - a;
+    err += (a - (s - bb)) + (b - bb);
+    return s;
+}
+
+// L += a.  COMP: the rounding errors are kept in Lc, so that L + Lc is the
+// sum of the addends to about eps^2 while L itself is bit-identical to the
+// plain sum.  The adjoint needs the radiance a path gathers after a bounce as
+// (total - gathered so far): in binary32 alone that difference loses all its
+// digits when the rest is small against the total.
+template <bool COMP>
+__device__ __forceinline__ void add_L(PathState& ps, f3 a) {
+    if (!COMP) {
+        ps.L = add3(ps.L, a);
+        return;
+    }
+    ps.L.x = two_sum(ps.L.x, a.x, ps.Lc.x);
+    ps.L.y = two_sum(ps.L.y, a.y, ps.Lc.y);
+    ps.L.z = two_sum(ps.L.z, a.z, ps.Lc.z);
+}
+
+template <bool COMP = false>
 __device__ __forceinline__ void resolve_nee(PathState& ps, const Nee& n, bool occ_a, bool occ_b) {
     if (!n.has) return;
     f3 Ld = (n.na && !occ_a) ? n.a : f3{0.0f, 0.0f, 0.0f};
     if (n.nb) Ld = add3(Ld, occ_b ? f3{0.0f, 0.0f, 0.0f} : n.b);
-    ps.L = add3(ps.L, mul3(n.beta, Ld));
+    add_L<COMP>(ps, mul3(n.beta, Ld));
 }
 
 // Returns true when the path continues with the ray stored in ps.
@@ -2446,7 +2473,7 @@ struct Tape {
 
 // TEX: image textures, bump and ray differentials; `rd` holds the
 // differentials of `ray` and receives those of the next ray.
-template <bool TEX = false>
+template <bool TEX = false, bool COMP = false>
 __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, const Ray& ray, bool hit, const Hit& h, Nee& nee,
                              float4* lookat, Tape* tp = nullptr, RayDiff* rd = nullptr) {
     const int rk = S.rng_kind;
@@ -2465,7 +2492,7 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
                 tp->esc_dir = nrm3(ray.d);
                 tp->esc_beta = ps.beta;
             }
-            ps.L = add3(ps.L, mul3(ps.beta, sky_eval(S, nrm3(ray.d))));
+            add_L<COMP>(ps, mul3(ps.beta, sky_eval(S, nrm3(ray.d))));
         }
         return false;
     }
@@ -2483,7 +2510,7 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, PathState& ps, c
     if (is_emitter(I) && (ps.bounce == 0 || ps.specular)) {
         const f3 Le = (I.type & NOOR_MAT_MESHLIGHT) ? light_Le(S, ray.d, (int)I.ins)
                                                     : ld3(TEX ? mt.emittance : S.mats[I.mat].emittance);
-        ps.L = add3(ps.L, mul3(ps.beta, Le));
+        add_L<COMP>(ps, mul3(ps.beta, Le));
         return false;
     }
     // accumulate: bump before the BSDF is built (noor/cuda/accumulate.cuh:84-99)

@@ -24,6 +24,7 @@ namespace noor {
 
 struct AdjArgs {
     const float4* radiance;  // per pixel: radiance of the forward sample (xyz)
+    const float4* radiance_lo;  // per pixel: rounding error of that sum (xyz), or null
     const float4* grad;      // per pixel: dLoss / dradiance (xyz)
     float* g_albedo;         // num_mats * 3, added to
     float* g_rough;          // num_mats, added to
@@ -92,8 +93,11 @@ __global__ __launch_bounds__(64) void k_adjoint(const DevScene* __restrict__ Sp,
         if (is_black(dl)) continue;  // no gradient flows through this pixel
         const float4 r4 = A.radiance[gid];
         const f3 Ltot = f3{r4.x, r4.y, r4.z};
+        const float4 l4 = A.radiance_lo ? A.radiance_lo[gid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const f3 Ltot_lo = f3{l4.x, l4.y, l4.z};
         PathState ps;
         ps.L = f3{0.0f, 0.0f, 0.0f};
+        ps.Lc = f3{0.0f, 0.0f, 0.0f};
         ps.beta = f3{1.0f, 1.0f, 1.0f};
         ps.bounce = 0;
         ps.specular = 0;
@@ -104,11 +108,11 @@ __global__ __launch_bounds__(64) void k_adjoint(const DevScene* __restrict__ Sp,
             const bool hit = bvh_closest<false>(S, ray, h, st, cnt, overflow);
             Nee nee;
             Tape tp;
-            const bool cont = shade_bounce(S, ps, ray, hit, h, nee, nullptr, &tp);
+            const bool cont = shade_bounce<false, true>(S, ps, ray, hit, h, nee, nullptr, &tp);
             if (tp.esc) add_env(S, genv, tp.esc_dir, mul3(dl, tp.esc_beta));
             bool oa = false;
             if (nee.na) oa = bvh_any<false>(S, nee.ra, st, cnt, overflow);
-            resolve_nee(ps, nee, oa, false);
+            resolve_nee<true>(ps, nee, oa, false);
             if (tp.nee || tp.scat) {
                 const Isect& I = tp.I;
                 const DevMat& m = S.mats[I.mat];
@@ -134,7 +138,9 @@ __global__ __launch_bounds__(64) void k_adjoint(const DevScene* __restrict__ Sp,
                 if (tp.scat && I.mat < A.num_mats) {
                     f3 dR, dr;
                     bsdf_grad(bs, I, m, I.wo, tp.scat_wi, B_ALL, tp.scat_lobe, dR, dr);
-                    const f3 rest = sub3(Ltot, ps.L);
+                    // the radiance gathered after this bounce: (total - gathered so
+                    // far), each as a compensated pair (sum, rounding error)
+                    const f3 rest = add3(sub3(Ltot, ps.L), sub3(Ltot_lo, ps.Lc));
                     const f3 f = tp.scat_f;
                     const f3 q = f3{f.x != 0.0f ? dl.x * rest.x / f.x : 0.0f, f.y != 0.0f ? dl.y * rest.y / f.y : 0.0f,
                                     f.z != 0.0f ? dl.z * rest.z / f.z : 0.0f};
@@ -188,12 +194,13 @@ hipError_t launch_set_params(const Context& c, const float* albedo, const float*
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(const Context& c, uint32_t stream, const float* radiance, const float* grad, float* g_albedo,
-                          float* g_rough, uint32_t num_mats, float* g_env) {
+hipError_t launch_adjoint(const Context& c, uint32_t stream, const float* radiance, const float4* radiance_lo,
+                          const float* grad, float* g_albedo, float* g_rough, uint32_t num_mats, float* g_env) {
     hipError_t se = sync_scene(c);
     if (se != hipSuccess) return se;
     AdjArgs A;
     A.radiance = (const float4*)radiance;
+    A.radiance_lo = radiance_lo;
     A.grad = (const float4*)grad;
     A.g_albedo = g_albedo;
     A.g_rough = g_rough;

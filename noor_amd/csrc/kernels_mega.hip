@@ -45,7 +45,8 @@ __device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext
 template <bool STATS, bool DIRECT, bool TEX = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_megakernel(const DevScene* __restrict__ Sp, float4* __restrict__ film, uint32_t frame0,
                                                    uint32_t spp, DevStats* __restrict__ stats,
-                                                   float4* __restrict__ lookat_out, float4* __restrict__ direct) {
+                                                   float4* __restrict__ lookat_out, float4* __restrict__ direct,
+                                                   float4* __restrict__ direct_lo) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     const uint32_t lane = threadIdx.x & 63;
@@ -70,6 +71,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             lookat = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             PathState ps;
             ps.L = f3{0.0f, 0.0f, 0.0f};
+            ps.Lc = f3{0.0f, 0.0f, 0.0f};
             ps.beta = f3{1.0f, 1.0f, 1.0f};
             ps.bounce = 0;
             ps.specular = 0;
@@ -81,7 +83,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 ++ext;
                 const bool hit = bvh_closest<STATS, TEX>(S, ray, h, st, cnt, overflow);
                 Nee nee;
-                const bool cont = shade_bounce<TEX>(S, ps, ray, hit, h, nee, center ? &lookat : nullptr, nullptr, &rd);
+                const bool cont =
+                    shade_bounce<TEX, DIRECT>(S, ps, ray, hit, h, nee, center ? &lookat : nullptr, nullptr, &rd);
                 bool oa = false, ob = false;
                 if (nee.na) {
                     ++shd;
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                     ++shd;
                     ob = bvh_any<STATS, TEX>(S, nee.rb, st, cns, overflow);
                 }
-                resolve_nee(ps, nee, oa, ob);
+                resolve_nee<DIRECT>(ps, nee, oa, ob);
                 if (!cont) break;
                 ray = mkray(ps.o, ps.d, K_INF);
                 ps.bounce++;
@@ -99,6 +102,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             if (DIRECT) {
                 const f3 c = final_radiance(S, ps.L);
                 acc = make_float4(c.x, c.y, c.z, 1.0f);
+                // the rounding error of the sample's sum (unclamped samples only)
+                if (direct_lo) direct_lo[gid] = make_float4(ps.Lc.x, ps.Lc.y, ps.Lc.z, 0.0f);
             } else {
                 acc = film_lerp(acc, final_radiance(S, ps.L), frame);
             }
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     flush_stats(stats, ext, shd, cnt, cns, overflow);
 }
 
-hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct) {
+hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct, float4* direct_lo) {
     hipError_t se = sync_scene(c);
     if (se != hipSuccess) return se;
     const uint32_t tiles = ((c.S.cam.w + 7) / 8) * ((c.S.cam.h + 7) / 8);
@@ -122,19 +127,19 @@ hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, fl
     if (direct && c.S.textured) return hipErrorNotSupported;
     if (direct)
         hipLaunchKernelGGL((k_megakernel<false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
-                           spp, c.d_stats, c.d_lookat, direct);
+                           spp, c.d_stats, c.d_lookat, direct, direct_lo);
     else if (c.S.textured && c.opt.stats)
         hipLaunchKernelGGL((k_megakernel<true, false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film,
-                           frame0, spp, c.d_stats, c.d_lookat, direct);
+                           frame0, spp, c.d_stats, c.d_lookat, direct, direct_lo);
     else if (c.S.textured)
         hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film,
-                           frame0, spp, c.d_stats, c.d_lookat, direct);
+                           frame0, spp, c.d_stats, c.d_lookat, direct, direct_lo);
     else if (c.opt.stats)
         hipLaunchKernelGGL((k_megakernel<true, false>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
-                           spp, c.d_stats, c.d_lookat, direct);
+                           spp, c.d_stats, c.d_lookat, direct, direct_lo);
     else
         hipLaunchKernelGGL((k_megakernel<false, false>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, c.film, frame0,
-                           spp, c.d_stats, c.d_lookat, direct);
+                           spp, c.d_stats, c.d_lookat, direct, direct_lo);
     T.end(c.stream);
     return hipGetLastError();
 }

@@ -70,7 +70,12 @@ struct WaveCtl {
     uint32_t pad[1];
 };
 
-struct WaveBuffers {
+// Device view of a context's path pool: device addresses and sizes only
+// (trivially copyable).  Kernels take it by value: pointers that arrive as
+// kernel arguments are known to be global, so their loads and stores compile
+// to global instructions; read from a copy in device memory they became flat
+// instructions and the wavefront step ran about 30% slower (measured).
+struct WavePool {
     uint32_t capacity = 0;
     int block = 64;
     int stack_lds = 0;
@@ -116,7 +121,12 @@ struct WaveBuffers {
     ViewDesc* views = nullptr;
     Segment* segs = nullptr;
     int32_t* seg_of = nullptr;    // [sample * num_views + view] -> segment of the group, -1 if none
-    // graph
+};
+
+// Host bookkeeping of the pool: the device view, the captured graph and the
+// allocations to free.
+struct WaveBuffers {
+    WavePool P;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_stats = false;
@@ -181,7 +191,7 @@ __device__ __forceinline__ int material_class(const DevScene& S, bool hit, const
     return 5;
 }
 
-__device__ __forceinline__ void load_state(const WaveBuffers& B, uint32_t p, PathState& ps) {
+__device__ __forceinline__ void load_state(const WavePool& B, uint32_t p, PathState& ps) {
     const float4 L = B.L[p], bt = B.beta[p];
     ps.L = f3{L.x, L.y, L.z};
     ps.beta = f3{bt.x, bt.y, bt.z};
@@ -191,7 +201,7 @@ __device__ __forceinline__ void load_state(const WaveBuffers& B, uint32_t p, Pat
     ps.rng.state = B.rng[p];
     ps.rng.inc = B.rng_inc[p];
 }
-__device__ __forceinline__ void store_state(const WaveBuffers& B, uint32_t p, const PathState& ps) {
+__device__ __forceinline__ void store_state(const WavePool& B, uint32_t p, const PathState& ps) {
     B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
     B.beta[p] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, __uint_as_float(ps.bounce | (ps.specular << 16)));
     B.rng[p] = ps.rng.state;
@@ -201,7 +211,7 @@ __device__ __forceinline__ void store_state(const WaveBuffers& B, uint32_t p, co
 // kernels
 
 // Slot pool set-up for a group of `items` items.
-__global__ void k_stream_init(WaveBuffers B, uint32_t items) {
+__global__ void k_stream_init(WavePool B, uint32_t items) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     WaveCtl* ctl = B.ctl;
     if (t < B.capacity) {
@@ -224,13 +234,13 @@ __global__ void k_stream_init(WaveBuffers B, uint32_t items) {
 // Retires the paths of the free queue and starts new items in their slots.
 // Items are handed out in free-queue order, so no atomics are needed: thread
 // t starts item item_next + t and appends its ray after the continuing rays.
-__device__ __forceinline__ void store_diff(const WaveBuffers& B, uint32_t p, const RayDiff& r) {
+__device__ __forceinline__ void store_diff(const WavePool& B, uint32_t p, const RayDiff& r) {
     B.dox[p] = make_float4(r.ox.x, r.ox.y, r.ox.z, __int_as_float(r.has));
     B.doy[p] = make_float4(r.oy.x, r.oy.y, r.oy.z, 0.0f);
     B.ddx[p] = make_float4(r.dx.x, r.dx.y, r.dx.z, 0.0f);
     B.ddy[p] = make_float4(r.dy.x, r.dy.y, r.dy.z, 0.0f);
 }
-__device__ __forceinline__ void load_diff(const WaveBuffers& B, uint32_t p, RayDiff& r) {
+__device__ __forceinline__ void load_diff(const WavePool& B, uint32_t p, RayDiff& r) {
     const float4 a = B.dox[p], b = B.doy[p], c = B.ddx[p], d = B.ddy[p];
     r.ox = f3{a.x, a.y, a.z};
     r.has = __float_as_int(a.w);
@@ -240,7 +250,7 @@ __device__ __forceinline__ void load_diff(const WaveBuffers& B, uint32_t p, RayD
 }
 
 template <bool TEX>
-__global__ void k_regen(const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
+__global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
     const DevScene& S = *Sp;
     WaveCtl* ctl = B.ctl;
     const int nxt = cur ^ 1;
@@ -332,7 +342,7 @@ __device__ __forceinline__ uint32_t refill(uint32_t* work, uint32_t count, bool 
 }
 
 template <bool STATS, bool ALPHA>
-__global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur, DevStats* stats) {
+__global__ void k_extend(const DevScene* __restrict__ Sp, WavePool B, int cur, DevStats* stats) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     WaveCtl* ctl = B.ctl;
@@ -371,7 +381,7 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WaveBuffers B, int cur
 
 // Material-class binning of the extension queue, in queue order: one LDS
 // histogram and one atomic per class per block.
-__global__ void k_bin(WaveBuffers B, int cur) {
+__global__ void k_bin(WavePool B, int cur) {
     WaveCtl* ctl = B.ctl;
     const uint32_t count = ctl->ray_count[cur] + ctl->new_rays[cur];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -418,7 +428,7 @@ __global__ void k_bin(WaveBuffers B, int cur) {
 // two waves; holding it to three waves spills 84 registers and gains nothing.
 template <bool TEX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4))) void k_shade(
-    const DevScene* __restrict__ Sp, WaveBuffers B, int cur) {
+    const DevScene* __restrict__ Sp, WavePool B, int cur) {
     const DevScene& S = *Sp;
     WaveCtl* ctl = B.ctl;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -497,7 +507,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
 // MIS: a task may hold a second, BSDF-sampled shadow ray (separate
 // instantiation, so the kernel without MIS keeps its registers)
 template <bool STATS, bool ALPHA, bool MIS>
-__global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStats* stats) {
+__global__ void k_shadow(const DevScene* __restrict__ Sp, WavePool B, DevStats* stats) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
     WaveCtl* ctl = B.ctl;
@@ -563,7 +573,7 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WaveBuffers B, DevStat
 }
 
 // Folds samples [s0, s1) of every pixel into its film, in sample order.
-__global__ void k_finish(WaveBuffers B, float4* films, uint32_t s0, uint32_t s1, uint32_t pixels) {
+__global__ void k_finish(WavePool B, float4* films, uint32_t s0, uint32_t s1, uint32_t pixels) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= pixels) return;
     const WaveCtl* ctl = B.ctl;
@@ -598,54 +608,55 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
         if (e == hipSuccess) B.allocs.push_back(*p);
         return e;
     };
+    WavePool& P = B.P;
     hipError_t e = hipSuccess;
 #define WA(ptr, n) \
     if (e == hipSuccess) e = alloc((void**)&(ptr), sizeof(*(ptr)) * (size_t)(n));
-    WA(B.item, cap);
-    WA(B.pix, cap);
-    WA(B.view, cap);
-    WA(B.L, cap);
-    WA(B.beta, cap);
-    WA(B.rng, cap);
-    WA(B.rng_inc, cap);
-    WA(B.ro, cap);
-    WA(B.rd, cap);
+    WA(P.item, cap);
+    WA(P.pix, cap);
+    WA(P.view, cap);
+    WA(P.L, cap);
+    WA(P.beta, cap);
+    WA(P.rng, cap);
+    WA(P.rng_inc, cap);
+    WA(P.ro, cap);
+    WA(P.rd, cap);
     if (tex) {
-        WA(B.dox, cap);
-        WA(B.doy, cap);
-        WA(B.ddx, cap);
-        WA(B.ddy, cap);
+        WA(P.dox, cap);
+        WA(P.doy, cap);
+        WA(P.ddx, cap);
+        WA(P.ddy, cap);
     }
-    B.tex = tex;
-    WA(B.hit, cap);
-    WA(B.hit2, cap);
-    WA(B.cls, cap);
-    WA(B.nee_a, cap);
-    WA(B.nee_beta, cap);
-    WA(B.sh_o, cap);
-    WA(B.sh_d, cap);
+    P.tex = tex;
+    WA(P.hit, cap);
+    WA(P.hit2, cap);
+    WA(P.cls, cap);
+    WA(P.nee_a, cap);
+    WA(P.nee_beta, cap);
+    WA(P.sh_o, cap);
+    WA(P.sh_d, cap);
     if (mis) {
-        WA(B.nee_b, cap);
-        WA(B.sh2_o, cap);
-        WA(B.sh2_d, cap);
+        WA(P.nee_b, cap);
+        WA(P.sh2_o, cap);
+        WA(P.sh2_d, cap);
     }
-    B.mis = mis;
-    WA(B.q_ray[0], cap);
-    WA(B.q_ray[1], cap);
-    WA(B.q_free[0], cap);
-    WA(B.q_free[1], cap);
-    WA(B.q_class, (size_t)kClasses * cap);
-    WA(B.q_shadow, cap);
-    WA(B.sb, sb_items);
-    WA(B.ctl, 1);
-    WA(B.views, max_views);
-    WA(B.segs, max_segments);
-    WA(B.seg_of, max_segments);
+    P.mis = mis;
+    WA(P.q_ray[0], cap);
+    WA(P.q_ray[1], cap);
+    WA(P.q_free[0], cap);
+    WA(P.q_free[1], cap);
+    WA(P.q_class, (size_t)kClasses * cap);
+    WA(P.q_shadow, cap);
+    WA(P.sb, sb_items);
+    WA(P.ctl, 1);
+    WA(P.views, max_views);
+    WA(P.segs, max_segments);
+    WA(P.seg_of, max_segments);
 #undef WA
-    B.capacity = cap;
-    B.sb_items = sb_items;
-    B.max_views = max_views;
-    B.max_segments = max_segments;
+    P.capacity = cap;
+    P.sb_items = sb_items;
+    P.max_views = max_views;
+    P.max_segments = max_segments;
     return e;
 }
 
@@ -661,10 +672,10 @@ void wave_free(Context& c) {
 
 // Records `iters` iterations (regen, extend, shade, shadow) on c.stream.
 static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
-    const uint32_t cap = B.capacity;
+    const uint32_t cap = B.P.capacity;
     const dim3 g1((cap + 255) / 256), b1(256);
-    const int blk = B.block;
-    const size_t lds = (size_t)blk * B.stack_lds * sizeof(uint32_t);
+    const int blk = B.P.block;
+    const size_t lds = (size_t)blk * B.P.stack_lds * sizeof(uint32_t);
     const dim3 gp(B.persist_waves / (blk / 64)), bp(blk);
     KernelTimer& T = c.timer;
     const bool tex = c.S.textured != 0;
@@ -679,19 +690,19 @@ static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
     for (int i = 0; i < iters; ++i) {
         const int cur = i & 1;
         T.begin(c.stream, "wave_regen");
-        hipLaunchKernelGGL(regen, g1, b1, 0, c.stream, c.d_scene, B, cur);
+        hipLaunchKernelGGL(regen, g1, b1, 0, c.stream, c.d_scene, B.P, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_extend");
-        hipLaunchKernelGGL(extend, gp, bp, lds, c.stream, c.d_scene, B, cur, c.d_stats);
+        hipLaunchKernelGGL(extend, gp, bp, lds, c.stream, c.d_scene, B.P, cur, c.d_stats);
         T.end(c.stream);
         T.begin(c.stream, "wave_bin");
-        hipLaunchKernelGGL(k_bin, g1, b1, 0, c.stream, B, cur);
+        hipLaunchKernelGGL(k_bin, g1, b1, 0, c.stream, B.P, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_shade");
-        hipLaunchKernelGGL(shade, g1, b1, 0, c.stream, c.d_scene, B, cur);
+        hipLaunchKernelGGL(shade, g1, b1, 0, c.stream, c.d_scene, B.P, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_shadow");
-        hipLaunchKernelGGL(shadow, gp, bp, lds, c.stream, c.d_scene, B, c.d_stats);
+        hipLaunchKernelGGL(shadow, gp, bp, lds, c.stream, c.d_scene, B.P, c.d_stats);
         T.end(c.stream);
     }
     return hipGetLastError();
@@ -737,13 +748,14 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
     if (capacity == 0) capacity = (uint32_t)std::min<uint64_t>(sb_need, 1u << 22);
     capacity = std::max<uint32_t>(256u, (capacity + 255) & ~255u);
     const uint32_t seg_need = (uint32_t)std::max<uint64_t>((uint64_t)max_spp * nv, 1);
-    if (!c.wave || c.wave->capacity != capacity || c.wave->sb_items < sb_need || c.wave->max_views < nv ||
-        c.wave->max_segments < seg_need || c.wave->block != block || c.wave->stack_lds != c.S.stack_lds ||
-        c.wave->tex != (c.S.textured != 0) || c.wave->mis != (c.S.mis != 0)) {
+    const WavePool* pool = c.wave ? &c.wave->P : nullptr;
+    if (!pool || pool->capacity != capacity || pool->sb_items < sb_need || pool->max_views < nv ||
+        pool->max_segments < seg_need || pool->block != block || pool->stack_lds != c.S.stack_lds ||
+        pool->tex != (c.S.textured != 0) || pool->mis != (c.S.mis != 0)) {
         wave_free(c);
         c.wave = new WaveBuffers();
-        c.wave->block = block;
-        c.wave->stack_lds = c.S.stack_lds;
+        c.wave->P.block = block;
+        c.wave->P.stack_lds = c.S.stack_lds;
         e = wave_alloc(*c.wave, capacity, (uint32_t)sb_need, std::max<uint32_t>(nv, 64), std::max<uint32_t>(seg_need, 256),
                        c.S.textured != 0, c.S.mis != 0);
         if (e != hipSuccess) {
@@ -760,7 +772,7 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
         B.exec = nullptr;
         B.graph = nullptr;
     }
-    e = hipMemcpyAsync(B.views, views.data(), sizeof(ViewDesc) * nv, hipMemcpyHostToDevice, c.stream);
+    e = hipMemcpyAsync(B.P.views, views.data(), sizeof(ViewDesc) * nv, hipMemcpyHostToDevice, c.stream);
     if (e != hipSuccess) return e;
     for (const auto& g : groups) {
         // segments of this group in item order (sample-major, then view)
@@ -777,19 +789,19 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
         WaveCtl ctl{};
         ctl.num_segments = (uint32_t)segs.size();
         ctl.num_views = nv;
-        e = hipMemcpyAsync(B.segs, segs.data(), sizeof(Segment) * segs.size(), hipMemcpyHostToDevice, c.stream);
+        e = hipMemcpyAsync(B.P.segs, segs.data(), sizeof(Segment) * segs.size(), hipMemcpyHostToDevice, c.stream);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(B.seg_of, seg_of.data(), sizeof(int32_t) * seg_of.size(), hipMemcpyHostToDevice,
+            e = hipMemcpyAsync(B.P.seg_of, seg_of.data(), sizeof(int32_t) * seg_of.size(), hipMemcpyHostToDevice,
                                c.stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(B.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, c.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(B.P.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, c.stream);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_stream_init, dim3((B.capacity + 255) / 256), dim3(256), 0, c.stream, B, items);
+        hipLaunchKernelGGL(k_stream_init, dim3((B.P.capacity + 255) / 256), dim3(256), 0, c.stream, B.P, items);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         // Every live path retires within `bounces` iterations, so within that
         // many iterations the whole pool is started afresh: the bound below is
         // never reached unless the device state is corrupt.
         const uint64_t max_iters =
-            2ull * ((uint64_t)items / B.capacity + 2) * (uint64_t)(std::max(c.S.bounces, 1) + 1) + 4 * kIters;
+            2ull * ((uint64_t)items / B.P.capacity + 2) * (uint64_t)(std::max(c.S.bounces, 1) + 1) + 4 * kIters;
         for (uint64_t it = 0;; it += kIters) {
             if (use_graph) {
                 if (!B.exec) {
@@ -809,7 +821,7 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
             }
             if (e != hipSuccess) return e;
             WaveCtl h{};
-            e = hipMemcpyAsync(&h, B.ctl, sizeof(h), hipMemcpyDeviceToHost, c.stream);
+            e = hipMemcpyAsync(&h, B.P.ctl, sizeof(h), hipMemcpyDeviceToHost, c.stream);
             if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
             if (e != hipSuccess) return e;
             // the last iteration of a replay has parity 1: its shade wrote the parity-0 queues
@@ -817,7 +829,7 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
             if (it > max_iters) return hipErrorUnknown;
         }
         c.timer.begin(c.stream, "wave_finish");
-        hipLaunchKernelGGL(k_finish, dim3((uint32_t)((pixels + 255) / 256)), dim3(256), 0, c.stream, B, films, g.first,
+        hipLaunchKernelGGL(k_finish, dim3((uint32_t)((pixels + 255) / 256)), dim3(256), 0, c.stream, B.P, films, g.first,
                            g.second, (uint32_t)pixels);
         c.timer.end(c.stream);
         if ((e = hipGetLastError()) != hipSuccess) return e;

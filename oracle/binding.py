@@ -18,7 +18,8 @@ _lib = None
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("extension_rays", "shadow_rays", "node_visits", "tri_tests",
-                                          "tri_second_fetch", "instance_entries")]
+                                          "tri_second_fetch", "instance_entries", "shd_node_visits",
+                                          "shd_tri_tests", "shd_tri_second_fetch", "shd_instance_entries")]
 
 
 def build():
@@ -52,6 +53,9 @@ def lib():
         f = getattr(L, f"oracle_render_{sfx}")
         f.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P(real),
                       P(Counters)]
+        f = getattr(L, f"oracle_render_sig_{sfx}")
+        f.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P(real),
+                      P(Counters), P(C.c_uint64)]
         f = getattr(L, f"oracle_trace_{sfx}")
         f.argtypes = [vp, P(abi.Ray), C.c_uint32, C.c_int, P(abi.Hit), P(Counters)]
     L.oracle_trace_brute_f32.argtypes = [vp, P(abi.Ray), C.c_uint32, P(abi.Hit)]
@@ -125,9 +129,11 @@ class OracleScene:
         r = None if rough is None else (C.c_float * 2)(*map(float, rough))
         lib().oracle_scene_set_material_params(self._h, m, d, r)
 
-    def render(self, frame0=1, spp=1, threads=0, rect=None, fp64=False, film=None):
+    def render(self, frame0=1, spp=1, threads=0, rect=None, fp64=False, film=None, signature=None):
         """Film (H, W, 4) after spp frames starting at frame0, plus counters.
-        A film passed in is continued (its running mean is updated in place)."""
+        A film passed in is continued (its running mean is updated in place).
+        `signature`: a uint64 (H, W) array that receives each pixel's path
+        signature (the hash of the discrete decisions of its last sample)."""
         if threads <= 0:
             threads = os.cpu_count() or 1
         x0, y0, x1, y1 = rect if rect else (0, 0, self.camera.w, self.camera.h)
@@ -142,7 +148,13 @@ class OracleScene:
         else:
             f = lib().oracle_render_f32
             ptr = film.ctypes.data_as(C.POINTER(C.c_float))
-        r = f(self._h, frame0, spp, threads, x0, y0, x1, y1, ptr, C.byref(cnt))
+        if signature is not None:
+            assert signature.shape == (h, w) and signature.dtype == np.uint64 and signature.flags.c_contiguous
+            f = lib().oracle_render_sig_f64 if fp64 else lib().oracle_render_sig_f32
+            r = f(self._h, frame0, spp, threads, x0, y0, x1, y1, ptr, C.byref(cnt),
+                  signature.ctypes.data_as(C.POINTER(C.c_uint64)))
+        else:
+            r = f(self._h, frame0, spp, threads, x0, y0, x1, y1, ptr, C.byref(cnt))
         if r != 0:
             raise RuntimeError(f"oracle render failed ({r})")
         return film, cnt

@@ -96,9 +96,15 @@ typedef struct orc_scene {
     int stack_size;
 } orc_scene;
 
+/* Traversal events of the reference's binary BVH, counted separately for
+   closest-hit (extension) and any-hit (shadow) traversals.  tri_second_fetch
+   counts the triangle tests whose Wald projection puts t in [0, tmax], i.e.
+   the tests at which the reference reads its second Wald record
+   (noor/cuda/triangle.cuh:100-104), whichever test decides the hit. */
 typedef struct orc_counters {
     uint64_t extension_rays, shadow_rays;
     uint64_t node_visits, tri_tests, tri_second_fetch, instance_entries;
+    uint64_t shd_node_visits, shd_tri_tests, shd_tri_second_fetch, shd_instance_entries;
 } orc_counters;
 
 /* scene preparation (oracle_scene.c) */
@@ -139,6 +145,12 @@ int oracle_render_f32(orc_scene* s, uint32_t frame0, uint32_t spp, int nthreads,
 int oracle_render_f64(orc_scene* s, uint32_t frame0, uint32_t spp, int nthreads,
                       uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
                       double* film, orc_counters* cnt);
+/* the same with each pixel's path signature (hash of its last sample's
+   discrete decisions) written to sig, row-major over the rectangle */
+int oracle_render_sig_f32(orc_scene* s, uint32_t frame0, uint32_t spp, int nthreads, uint32_t x0, uint32_t y0,
+                          uint32_t x1, uint32_t y1, float* film, orc_counters* cnt, uint64_t* sig);
+int oracle_render_sig_f64(orc_scene* s, uint32_t frame0, uint32_t spp, int nthreads, uint32_t x0, uint32_t y0,
+                          uint32_t x1, uint32_t y1, double* film, orc_counters* cnt, uint64_t* sig);
 int oracle_trace_f32(orc_scene* s, const noor_ray* rays, uint32_t n, int any_hit,
                      noor_hit* hits, orc_counters* cnt);
 int oracle_trace_f64(orc_scene* s, const noor_ray* rays, uint32_t n, int any_hit,

