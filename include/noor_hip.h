@@ -320,6 +320,16 @@ noor_status noor_synchronize(noor_ctx* ctx);
 /* Closest-hit (any_hit = 0) or any-hit (any_hit = 1) queries on host arrays. */
 noor_status noor_trace(noor_ctx* ctx, const noor_ray* rays, uint32_t n, int any_hit,
                        noor_hit* hits);
+/* noor_trace without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (graph capture, serving loops).  Rays and
+ * hits are DEVICE arrays of n entries; the call only enqueues the queries on
+ * the context's stream (results are ready after noor_synchronize).  The
+ * traversal stack beyond its LDS part uses a 16 MiB scratch array fixed in the
+ * library, one per GPU: calls that overlap on one GPU (several contexts or
+ * threads) race on it.  NOOR_ERR_UNSUPPORTED when one wave's stack does not fit
+ * (more than 65,536 device entries per lane); no allocation is made. */
+noor_status noor_trace_noalloc(noor_ctx* ctx, const noor_ray* d_rays, uint32_t n, int any_hit,
+                               noor_hit* d_hits);
 
 /* Live kernel timing with HIP events on the context's stream (per kernel
  * name: launches and total milliseconds).  Timing makes the wavefront

@@ -65,6 +65,7 @@ def lib():
         "noor_render_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
         "noor_adjoint": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.c_void_p]),
+        "noor_trace_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
         "noor_wide_bvh_check": (C.c_int, [P(abi.PayloadView), P(C.c_uint32), P(C.c_uint32)]),
         "noor_abi_struct_size": (C.c_size_t, [C.c_char_p]),
         "noor_scene_build": (C.c_int, [P(abi.SceneDesc), P(abi.BvhParams), P(C.c_void_p)]),

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -27,6 +28,39 @@ using noor::Context;
 struct noor_ctx {
     Context c;
 };
+
+// Traversal-stack scratch of noor_trace_noalloc: the stack entries beyond a
+// lane's LDS part, lane-interleaved, for callers that cannot allocate device
+// memory.  One copy per GPU, held until the process ends.  Host code never
+// passes this name to a kernel (on the host it is a stand-in, not a device
+// address): each device's address is looked up once, with that device
+// current, at the first noor_create on it (scratch_address below).
+constexpr size_t kScratchWords = (size_t)4 << 20;  // 16 MiB
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per
+// device), or nullptr with the HIP error in *err.
+uint32_t* scratch_address(int device, hipError_t* err) {
+    *err = hipSuccess;
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = (uint32_t*)p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
 
 namespace {
 
@@ -324,6 +358,12 @@ noor_status noor_create(const noor_payload_view* P, const noor_hosek* hosek, con
     if (hipGetDevice(&c->device) != hipSuccess) return bail(NOOR_ERR_NO_DEVICE);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(c, NOOR_ERR_HIP, "stream creation failed"));
+    {
+        // this device's copy of the no-allocation scratch (set up once per device)
+        hipError_t se = hipSuccess;
+        c->scratch = scratch_address(c->device, &se);
+        if (!c->scratch) return bail(fail(c, NOOR_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(se)));
+    }
 
     // walk of the binary BVH: detects out-of-range links and cycles
     std::vector<int32_t> memo(P->num_bvh_nodes, -1);
@@ -933,6 +973,26 @@ noor_status noor_trace(noor_ctx* ctx, const noor_ray* rays, uint32_t n, int any_
     hipFree(dr);
     hipFree(dh);
     if (e != hipSuccess) return fail(c, NOOR_ERR_HIP, std::string("trace: ") + hipGetErrorString(e));
+    return NOOR_OK;
+}
+
+noor_status noor_trace_noalloc(noor_ctx* ctx, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits) {
+    if (!ctx || (n && (!d_rays || !d_hits))) return NOOR_ERR_INVALID_ARG;
+    if (n == 0) return NOOR_OK;
+    if (n > (1u << 31)) return NOOR_ERR_INVALID_ARG;
+    Context* c = &ctx->c;
+    if (!c->scratch) return fail(c, NOOR_ERR_HIP, "trace_noalloc: no scratch on this context's device");
+    // the lanes whose deep stack entries fit the scratch
+    const size_t deep = (size_t)std::max(0, c->S.stack_depth - c->S.stack_lds);
+    uint32_t lanes = c->S.gstack_lanes;
+    if (deep) lanes = (uint32_t)std::min<size_t>(lanes, kScratchWords / deep);
+    lanes &= ~63u;
+    if (lanes < 64)
+        return fail(c, NOOR_ERR_UNSUPPORTED,
+                    "trace_noalloc: the BVH needs " + std::to_string(deep) +
+                        " device stack entries per lane, more than the 16 MiB scratch holds for one wave "
+                        "(raise the LDS part with noor_set_stack_lds)");
+    HIP_TRY(c, noor::launch_trace(*c, d_rays, n, any_hit, d_hits, c->scratch, lanes));
     return NOOR_OK;
 }
 

@@ -112,6 +112,7 @@ struct Context {
     uint32_t bvh_root = 0;        // the payload's binary root (the spec must name it)
     uint32_t num_materials = 0;
     uint32_t* gstack = nullptr;   // device part of the traversal stack
+    uint32_t* scratch = nullptr;  // this device's no-allocation scratch (not owned)
     // multi-view batch films
     float4* batch_film = nullptr;
     std::vector<uint32_t> batch_w, batch_h;
@@ -136,7 +137,10 @@ inline hipError_t sync_scene(const Context& cc) {
 // radiance sum, which the adjoint uses to recover the radiance after a bounce
 hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct = nullptr,
                              float4* direct_lo = nullptr);
-hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits);
+// `gstack`, `glanes`: the device part of the traversal stack to use instead of
+// the context's (lane-interleaved, (stack depth - LDS entries) x glanes words)
+hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits,
+                        uint32_t* gstack = nullptr, uint32_t glanes = 0);
 hipError_t launch_debug_skydome(const Context& c, uint32_t frame);
 // kernels_env.hip
 hipError_t launch_sky_bake(const Context& c, float4* out, int w, int h);

@@ -552,16 +552,21 @@ struct TravStack {
         return d < K ? lds[d * lstride] : glob[(size_t)(d - K) * gstride];
     }
 };
-// Stack of this lane; the launch must use at most S.gstack_lanes lanes.
-__device__ __forceinline__ TravStack make_stack(const DevScene& S, uint32_t* smem) {
+// Stack of this lane whose entries beyond the LDS part live in `gstack`,
+// lane-interleaved over `glanes` lanes; the launch must use at most `glanes`.
+__device__ __forceinline__ TravStack make_stack(const DevScene& S, uint32_t* smem, uint32_t* gstack, uint32_t glanes) {
     TravStack t;
     t.lds = (NOOR_LDS uint32_t*)(smem + threadIdx.x);
     t.lstride = (int)blockDim.x;
     t.K = S.stack_lds;
-    t.glob = (NOOR_GLOBAL uint32_t*)(S.gstack + (blockIdx.x * blockDim.x + threadIdx.x));
-    t.gstride = S.gstack_lanes;
+    t.glob = (NOOR_GLOBAL uint32_t*)(gstack + (blockIdx.x * blockDim.x + threadIdx.x));
+    t.gstride = glanes;
     t.depth = S.stack_depth;
     return t;
+}
+// The context's stack; the launch must use at most S.gstack_lanes lanes.
+__device__ __forceinline__ TravStack make_stack(const DevScene& S, uint32_t* smem) {
+    return make_stack(S, smem, S.gstack, S.gstack_lanes);
 }
 
 __device__ __forceinline__ uint32_t ref_of(const uint4& r, int i) {

@@ -145,12 +145,15 @@ hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, fl
 }
 
 // noor_trace: closest or any hit for a batch of rays.
+// `gstack`, `glanes`: the device part of the traversal stack and the lanes it
+// covers (the context's, or a caller's scratch); the grid uses at most glanes.
 template <bool STATS, bool ALPHA = false>
 __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, const noor_ray* __restrict__ rays, uint32_t n,
-                                              int any_hit, noor_hit* __restrict__ hits, DevStats* stats) {
+                                              int any_hit, noor_hit* __restrict__ hits, DevStats* stats,
+                                              uint32_t* gstack, uint32_t glanes) {
     const DevScene& S = *Sp;
     extern __shared__ uint32_t smem[];
-    const TravStack st = make_stack(S, smem);
+    const TravStack st = make_stack(S, smem, gstack, glanes);
     Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
     bool overflow = false;
     unsigned long long ext = 0, shd = 0;
@@ -185,23 +188,21 @@ __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, c
     flush_stats(stats, ext, shd, cnt, cns, overflow);
 }
 
-hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits) {
+hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits,
+                        uint32_t* gstack, uint32_t glanes) {
     hipError_t se = sync_scene(c);
     if (se != hipSuccess) return se;
+    if (!gstack) {
+        gstack = c.S.gstack;
+        glanes = c.S.gstack_lanes;
+    }
+    if (glanes < 64) return hipErrorInvalidValue;
     const size_t lds = (size_t)64 * c.S.stack_lds * sizeof(uint32_t);
-    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((n + 63) / 64, c.S.gstack_lanes / 64));
-    if (c.S.textured && c.opt.stats)
-        hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit,
-                           d_hits, c.d_stats);
-    else if (c.S.textured)
-        hipLaunchKernelGGL((k_trace<false, true>), dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit,
-                           d_hits, c.d_stats);
-    else if (c.opt.stats)
-        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit, d_hits,
-                           c.d_stats);
-    else
-        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit, d_hits,
-                           c.d_stats);
+    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((n + 63) / 64, glanes / 64));
+    auto k = c.S.textured ? (c.opt.stats ? k_trace<true, true> : k_trace<false, true>)
+                          : (c.opt.stats ? k_trace<true> : k_trace<false>);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(64), lds, c.stream, c.d_scene, d_rays, n, any_hit, d_hits, c.d_stats,
+                       gstack, glanes);
     return hipGetLastError();
 }
 
