@@ -237,7 +237,7 @@ __device__ __forceinline__ float rng_next(Rng& r, int kind) {
 struct Ray {
     f3 o, d, inv;
     float tmax;
-    uint32_t pn;  // bit a set when d[a] <= 0 (the reference posneg)
+    uint32_t pn;  // bit a set when d[a] <= 0 (the reference posneg); bit 3: box4_hit's fast form applies
     int kx, ky, kz;
     float Sx, Sy, Sz;
 };
@@ -245,6 +245,10 @@ struct Ray {
 __device__ __forceinline__ void ray_update(Ray& r) {
     r.inv = f3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
     r.pn = (r.d.x > 0.0f ? 0u : 1u) | (r.d.y > 0.0f ? 0u : 2u) | (r.d.z > 0.0f ? 0u : 4u);
+    // the fast slab form needs finite reciprocals whose products with a
+    // node's step cannot overflow (steps stay far below 2^40)
+    const float mi = fmaxf(fmaxf(fabsf(r.inv.x), fabsf(r.inv.y)), fabsf(r.inv.z));
+    if (mi <= 0x1p80f) r.pn |= 8u;
     // Woop, Benthin & Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013
     const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
     int kz = 0;
@@ -347,9 +351,11 @@ __device__ __forceinline__ float q_plane(float org, uint32_t q, int c, float s) 
     return __builtin_fmaf((float)((q >> (8 * c)) & 0xFFu), s, org);
 }
 
-// Slab test of the four children of a compressed node.  Bit c of the result
-// is set when child c is entered; tn[c] is its entry distance.
-__device__ __forceinline__ uint32_t box4_hit(const Ray& r, const WideNode& n, float tn[4]) {
+// Slab test of the four children of a compressed node, for rays with a zero
+// (or subnormal) direction component: each plane is decoded as the builder
+// checked it (q_plane) and measured from the ray origin, the reference's
+// arithmetic (noor/cuda/bbox.cuh:62-80) including its infinities.
+__device__ __forceinline__ uint32_t box4_hit_exact(const Ray& r, const WideNode& n, float tn[4]) {
     const float ox = __uint_as_float(n.a.x), oy = __uint_as_float(n.a.y), oz = __uint_as_float(n.a.z);
     const uint32_t meta = n.a.w;
     const float sx = __uint_as_float((meta & 0xFFu) << 23);
@@ -371,6 +377,52 @@ __device__ __forceinline__ uint32_t box4_hit(const Ray& r, const WideNode& n, fl
         const float tz1 = (q_plane(oz, fz, c, sz) - r.o.z) * r.inv.z;
         const float t0 = fmaxf(fmaxf(tx0, ty0), tz0);
         const float t1 = fminf(fminf(tx1, ty1), tz1) * BOX_SLACK;
+        tn[c] = t0;
+        if (t0 <= t1 && t0 <= tmaxs && t1 >= 0.0f) mask |= 1u << c;
+    }
+    return mask & (meta >> 24);
+}
+
+// Slab test of the four children of a compressed node.  Bit c of the result
+// is set when child c is entered; tn[c] is its entry distance.
+//
+// Per axis, A = (org - o) inv and B = s inv (exact: s is a power of two), and
+// the distance of plane q is one fused multiply-add, fma(q, B, A): two VALU
+// operations per plane (byte extract, fma) instead of four.  Against the
+// exact distance t* = (org + q s - o) / d of the exact grid value (which the
+// builder checked contains the child, q_exact in wide_bvh.cpp) it errs by a
+// relative (1 + d0)(1 + d3), from inv and the fma, and an absolute at most
+// 2.01 u |A| (u = 2^-24), from the rounding of org - o and of its product
+// with inv (q B is exact inside the fma).  The relative part is covered by
+// the 2^-20 relative widening of exit distances and tmax; the absolute part by
+// W = 2^-22 max|A|, subtracted from entry and added to exit distances.  So a
+// box holding a point of the ray between 0 and tmax is always entered.
+__device__ __forceinline__ uint32_t box4_hit(const Ray& r, const WideNode& n, float tn[4]) {
+    if (!(r.pn & 8u)) return box4_hit_exact(r, n, tn);
+    const uint32_t meta = n.a.w;
+    const float sx = __uint_as_float((meta & 0xFFu) << 23);
+    const float sy = __uint_as_float(((meta >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((meta >> 16) & 0xFFu) << 23);
+    const float Ax = (__uint_as_float(n.a.x) - r.o.x) * r.inv.x, Bx = sx * r.inv.x;
+    const float Ay = (__uint_as_float(n.a.y) - r.o.y) * r.inv.y, By = sy * r.inv.y;
+    const float Az = (__uint_as_float(n.a.z) - r.o.z) * r.inv.z, Bz = sz * r.inv.z;
+    const float W = fmaxf(fmaxf(fabsf(Ax), fabsf(Ay)), fabsf(Az)) * 0x1p-22f;
+    const uint32_t qlx = n.b.x, qly = n.b.y, qlz = n.b.z, qhx = n.b.w, qhy = n.c.x, qhz = n.c.y;
+    const uint32_t nx = (r.pn & 1u) ? qhx : qlx, fx = (r.pn & 1u) ? qlx : qhx;
+    const uint32_t ny = (r.pn & 2u) ? qhy : qly, fy = (r.pn & 2u) ? qly : qhy;
+    const uint32_t nz = (r.pn & 4u) ? qhz : qlz, fz = (r.pn & 4u) ? qlz : qhz;
+    const float tmaxs = r.tmax * BOX_SLACK;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float tx0 = __builtin_fmaf((float)((nx >> (8 * c)) & 0xFFu), Bx, Ax);
+        const float tx1 = __builtin_fmaf((float)((fx >> (8 * c)) & 0xFFu), Bx, Ax);
+        const float ty0 = __builtin_fmaf((float)((ny >> (8 * c)) & 0xFFu), By, Ay);
+        const float ty1 = __builtin_fmaf((float)((fy >> (8 * c)) & 0xFFu), By, Ay);
+        const float tz0 = __builtin_fmaf((float)((nz >> (8 * c)) & 0xFFu), Bz, Az);
+        const float tz1 = __builtin_fmaf((float)((fz >> (8 * c)) & 0xFFu), Bz, Az);
+        const float t0 = fmaxf(fmaxf(tx0, ty0), tz0) - W;
+        const float t1 = fminf(fminf(tx1, ty1), tz1) * BOX_SLACK + W;
         tn[c] = t0;
         if (t0 <= t1 && t0 <= tmaxs && t1 >= 0.0f) mask |= 1u << c;
     }

@@ -26,6 +26,12 @@ float q_decode(float org, uint32_t q, float s) {
     return org + qs;
 }
 
+// The grid value org + q s as a real number, which the GPU's fast slab test
+// (box4_hit) measures distances from.  Exact in binary64: a non-empty box is at
+// least one binary32 ulp of its planes wide, so s >= ulp(org) 2^-9 and the sum
+// spans fewer than 53 bits; an empty extent has q s = 0.
+double q_exact(float org, uint32_t q, float s) { return (double)org + (double)q * (double)s; }
+
 // Encodes one axis of one node: the smallest power-of-two step for which every
 // slot's rounded-out planes fit in 8 bits.  Returns false if none does.
 bool quantize_axis(const WNode& n, int a, uint32_t valid, float& org, uint32_t& ebyte, uint32_t& qlo, uint32_t& qhi) {
@@ -49,14 +55,21 @@ bool quantize_axis(const WNode& n, int a, uint32_t valid, float& org, uint32_t& 
             const float cl = n.lo[a][c], ch = n.hi[a][c];
             double ql = std::floor(((double)cl - (double)org) / (double)s);
             int64_t q = (int64_t)std::max(0.0, std::min(255.0, ql));
-            while (q > 0 && q_decode(org, (uint32_t)q, s) > cl) --q;
-            if (q_decode(org, (uint32_t)q, s) > cl) ok = false;
+            // both decodes (the binary32 one and the exact grid value) below the plane
+            auto above = [&](int64_t v) {
+                return q_decode(org, (uint32_t)v, s) > cl || q_exact(org, (uint32_t)v, s) > (double)cl;
+            };
+            auto below = [&](int64_t v) {
+                return q_decode(org, (uint32_t)v, s) < ch || q_exact(org, (uint32_t)v, s) < (double)ch;
+            };
+            while (q > 0 && above(q)) --q;
+            if (above(q)) ok = false;
             int64_t r = (int64_t)std::max(0.0, std::ceil(((double)ch - (double)org) / (double)s));
             if (r > 255) {
                 ok = false;
                 break;
             }
-            while (r <= 255 && q_decode(org, (uint32_t)r, s) < ch) ++r;
+            while (r <= 255 && below(r)) ++r;
             if (r > 255 || r < q) {
                 ok = false;
                 break;
@@ -502,9 +515,12 @@ std::string check_wide_bvh(const noor_payload_view* P, const WideBvh& W) {
             if (!valid) continue;
             for (int a = 0; a < 3; ++a) {
                 const float s = std::ldexp(1.0f, (int)((q.meta >> (8 * a)) & 0xFFu) - 127);
-                const float lo = q_decode(q.org[a], (q.qlo[a] >> (8 * c)) & 0xFFu, s);
-                const float hi = q_decode(q.org[a], (q.qhi[a] >> (8 * c)) & 0xFFu, s);
+                const uint32_t ql = (q.qlo[a] >> (8 * c)) & 0xFFu, qh = (q.qhi[a] >> (8 * c)) & 0xFFu;
+                const float lo = q_decode(q.org[a], ql, s);
+                const float hi = q_decode(q.org[a], qh, s);
                 if (!(lo <= n.lo[a][c] && hi >= n.hi[a][c])) return "compressed box does not contain its child";
+                if (!(q_exact(q.org[a], ql, s) <= (double)n.lo[a][c] && q_exact(q.org[a], qh, s) >= (double)n.hi[a][c]))
+                    return "compressed box (exact grid values) does not contain its child";
             }
         }
     }
