@@ -1,34 +1,51 @@
 """Benchmark of the render path on one MI355X.
 
-Workload: the 5M-triangle environment-lit terrain (1582^2 fBm heightfield,
-8 octaves, amplitude 20, + 1000 instanced GGX spheres, 2048x1024 log-normal
-HDR environment with three 2-degree hot spots), 1920x1080, 32 spp, 8 bounces.
-A step renders the full 32-spp frame (66.4M pixel samples) from frame 1.
+Headline workload (C2): the 5M-triangle environment-lit terrain (1582^2 fBm
+heightfield, 8 octaves, amplitude 20, + 1000 instanced GGX spheres, 2048x1024
+log-normal HDR environment with three 2-degree hot spots), 1920x1080, 32 spp,
+8 bounces.  A step renders the full 32-spp frame (66.4M pixel samples) from
+frame 1.
 
 Reported `value`: wavefront Mrays/s (extension + shadow rays) over the timed
 steps; the megakernel is timed the same way and reported beside it.  Ray
 counts come from the device counters of an untimed counting pass (they are
 deterministic: both renderers trace the same rays).
 
-`roofline`: the dominant kernel of the wavefront renderer, with its
-algorithmic bytes per launch (112 B per wide BVH node visit: four child boxes
-and references, 48 B per watertight triangle test, 48 B per instance entry,
-plus its per-ray queue and record traffic) divided by its average launch duration, measured with HIP events on
-the renderer's own stream during an extra, event-timed step.
+`roofline`: the dominant kernel of the wavefront renderer (the closest-hit
+traversal) against the HBM peak.  `achieved` = the reference-layout
+algorithmic bytes of one launch divided by its average duration (HIP events
+on the renderer's own stream, an extra event-timed step).  Reference-layout
+bytes count the events of the reference's binary-BVH traversal of the same
+rays, from the oracle's counters over the CPU sample: 32 B per node visit,
+32 B per triangle test plus 16 B when the second Wald record is read, 48 B per
+instance entry (per extension ray, scaled to the step's extension rays).
+`achieved_build_layout` counts this build's own fetches instead (56 B per
+compressed wide-node visit, 48 B per watertight triangle test, 48 B per
+instance entry, 80 B of queue and record traffic per ray).  `traffic` and
+`limiter` are the HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, scaled by a
+copy of known size) and the VALU activity of the same kernel, read from
+profiles/pmc_traffic_terrain_1080p.md when that profile was measured on the
+current kernel sources (matched by a hash of noor_amd/csrc, include and the
+build flags), else null.
 
 `cpu_baseline`: the CPU restatement of the same path (the oracle, FP32) on
 all host cores over a bounded sample of the same frame (a band of rows, or
 the whole frame at several spp, about 15 s of CPU work), in Mrays/s; the
 sample's pixels are compared with the GPU's film of the same rows and frames.
 
-`batch`: 48 views of the 1M-triangle Hosek-sky scene rendered as one
-wavefront batch (autotuned) against the megakernel view by view.
-`adjoint`: forward and path-replay backward times of the 512^2 one-sample
-differentiable render (bwd/fwd ratio).
+`configs`: the other configurations at their stated sizes: C0 (Cornell +
+81,920-triangle displaced sphere, 512^2, 16 spp; the CPU baseline over the
+whole frame, and the FP64 oracle at 256 spp on a crop with the RMSE), C1 (1M
+triangles, Hosek sky, 1920x1080, 64 spp, with a CPU sample), C3 (`batch`: 48
+views of the C1 scene as one autotuned wavefront batch against the megakernel
+view by view), C4 (`adjoint`: forward and path-replay backward times of the
+512^2 one-sample differentiable render, bwd/fwd ratio).
 """
 import argparse
+import hashlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -38,23 +55,58 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s + spp/s @1080p (wavefront vs megakernel); bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
 WIDE_NODE_BYTES = 56  # per wide-node visit: origin + steps (16 B), 8-bit child planes (24 B), references (16 B)
-# HBM bytes per dispatch of the dominant kernel on the default terrain workload,
-# from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB
-# calibration copy (commands and figures in profiles/pmc_traffic_terrain_1080p.md)
-MEASURED_HBM_BYTES = {("terrain", 1920, 1080, 32, "wave_extend"): 1154551032}
+PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_terrain_1080p.md")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def source_hash():
+    """sha256 of the kernel and library sources and the build script: a
+    measured profile applies to the binary these produce."""
+    h = hashlib.sha256()
+    files = []
+    for d in ("noor_amd/csrc", "include"):
+        for f in sorted(os.listdir(os.path.join(ROOT, d))):
+            files.append(os.path.join(d, f))
+    files.append("noor_amd/build.py")
+    for f in files:
+        h.update(f.encode())
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def measured_profile():
+    """The JSON block of the PMC profile if it was measured on these sources."""
+    try:
+        text = open(PMC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\n(\{.*?\})\n```", text, re.S)
+    if not m:
+        return None
+    d = json.loads(m.group(1))
+    return d if d.get("source_sha256") == source_hash() else None
+
+
 def traversal_bytes(st, prefix, tri_test):
+    """This build's own fetches (compressed wide nodes)."""
     nodes = st[prefix + "node_visits"]
     tris = st[prefix + "tri_tests"]
     inst = st[prefix + "instance_entries"]
     second = st[prefix + "tri_second_fetch"]
     tri_bytes = 48 * tris if tri_test == 0 else 32 * tris + 16 * second
     return WIDE_NODE_BYTES * nodes + tri_bytes + 48 * inst
+
+
+def reference_bytes(cnt, shadow=False):
+    """Reference-layout traversal bytes of the oracle's counted events (the
+    reference's binary BVH and Wald records)."""
+    p = "shd_" if shadow else ""
+    return (32 * getattr(cnt, p + "node_visits") + 32 * getattr(cnt, p + "tri_tests") +
+            16 * getattr(cnt, p + "tri_second_fetch") + 48 * getattr(cnt, p + "instance_entries"))
 
 
 def cpu_baseline(wl, target_s):
@@ -86,17 +138,111 @@ def cpu_baseline(wl, target_s):
         "sample": f"rows {y0}-{y0 + want_rows} of the {W}x{H} frame at {spp} spp (frames 1-{spp}), "
                   f"FP32 oracle, {rays} rays in {dt:.1f} s",
         "spp_per_s": round(want_rows * W * spp / dt, 1),
-    }, film, (y0, y0 + want_rows), spp
+    }, film, (y0, y0 + want_rows), spp, cnt
 
 
-def bench_batch(log_prefix="batch"):
+def _count_rays(r, wl, go, spp):
+    """Rays of one render (device counters, an untimed counting pass)."""
+    wl.options.stats = 1
+    r.update_options(wl.options)
+    r.reset_stats()
+    go(spp=spp, frame=1)
+    st = r.stats()
+    wl.options.stats = 0
+    r.update_options(wl.options)
+    return st["extension_rays"], st["shadow_rays"]
+
+
+def _time_render(r, go, spp, reps):
+    go(spp=spp, frame=1)  # warm-up
+    r.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        go(spp=spp, frame=1)
+    r.synchronize()
+    return (time.perf_counter() - t) / reps
+
+
+def bench_config(wl, name, cpu_seconds, reps, cpu_full=False):
+    """One configuration at its stated size: megakernel and wavefront Mrays/s
+    and spp/s, their films compared, and the CPU baseline (the whole frame when
+    `cpu_full`, else a bounded sample) compared pixel by pixel with the GPU."""
+    import numpy as np
+    from noor_amd.renderer import Renderer
+    from oracle.binding import OracleScene
+    r = Renderer.from_workload(wl)
+    spp = wl.spp
+    ext, shd = _count_rays(r, wl, r.render_wavefront, spp)
+    rays = ext + shd
+    px = wl.camera.w * wl.camera.h
+    row = {"workload": f"{name}: {wl.payload.view.num_triangles} tris, {wl.camera.w}x{wl.camera.h}, {spp} spp, "
+                       f"{wl.spec.bounces} bounces", "rays_per_render": {"extension": ext, "shadow": shd}}
+    films = {}
+    for mode, go in (("megakernel", r.render), ("wavefront", r.render_wavefront)):
+        dt = _time_render(r, go, spp, reps)
+        films[mode] = r.film()
+        row[mode] = {"ms": round(dt * 1e3, 2), "mrays_per_s": round(rays / dt / 1e6, 1),
+                     "spp_per_s": round(px * spp / dt, 1)}
+    row["wavefront_vs_megakernel"] = round(row["wavefront"]["mrays_per_s"] / row["megakernel"]["mrays_per_s"], 3)
+    row["wavefront_equals_megakernel"] = bool(np.array_equal(films["megakernel"], films["wavefront"]))
+    if cpu_seconds > 0 or cpu_full:
+        cores = os.cpu_count() or 1
+        if cpu_full:
+            t = time.time()
+            cf, cnt = OracleScene.from_workload(wl).render(1, spp, threads=cores)
+            dt = time.time() - t
+            cr = cnt.extension_rays + cnt.shadow_rays
+            row["cpu_baseline"] = {"value": round(cr / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores,
+                                   "kind": "port", "sample": f"the whole frame at {spp} spp, FP32 oracle, "
+                                                             f"{cr} rays in {dt:.1f} s",
+                                   "spp_per_s": round(px * spp / dt, 1)}
+            g = films["megakernel"]
+        else:
+            cb, cf, rows, cspp, _ = cpu_baseline(wl, cpu_seconds)
+            row["cpu_baseline"] = cb
+            r.render(spp=cspp, frame=1)
+            g = r.film()[rows[0]:rows[1]]
+        row["cpu_vs_gpu_identical_pixels"] = float(np.mean(np.all(g == cf, axis=-1)))
+        row["cpu_vs_gpu_mean_rel_err"] = float(np.mean(np.abs(g[..., :3].astype(np.float64) - cf[..., :3]) /
+                                                       (np.abs(cf[..., :3]) + 1e-3)))
+    log(f"{name}: megakernel {row['megakernel']['mrays_per_s']} Mrays/s, wavefront {row['wavefront']['mrays_per_s']} "
+        f"Mrays/s, films equal {row['wavefront_equals_megakernel']}"
+        + (f", cpu {row['cpu_baseline']['value']} Mrays/s" if "cpu_baseline" in row else ""))
+    r.close()
+    return row
+
+
+def bench_fp64_rmse(wl, spp=256, rect=(208, 288, 304, 384)):
+    """The GPU's FP32 film against the FP64 oracle on the same RNG streams, at
+    high spp on a crop: RMSE, relative RMSE and mean relative error."""
+    import numpy as np
+    from noor_amd.renderer import Renderer
+    from oracle.binding import OracleScene
+    r = Renderer.from_workload(wl)
+    r.render(spp=spp, frame=1)
+    g = r.film()[..., :3].astype(np.float64)
+    r.close()
+    t = time.time()
+    f64, _ = OracleScene.from_workload(wl).render(1, spp, rect=rect, fp64=True)
+    dt = time.time() - t
+    x0, y0, x1, y1 = rect
+    crop, ref = g[y0:y1, x0:x1], f64[..., :3]
+    rmse = float(np.sqrt(np.mean((crop - ref) ** 2)))
+    out = {"spp": spp, "crop": list(rect), "rmse": rmse, "relative_rmse": rmse / float(np.sqrt(np.mean(ref ** 2))),
+           "mean_rel_err": float(np.mean(np.abs(crop - ref) / (np.abs(ref) + 1e-3))), "fp64_seconds": round(dt, 1)}
+    log(f"FP64 oracle: RMSE {rmse:.3e} (relative {out['relative_rmse']:.3e}) at {spp} spp on {rect}")
+    return out
+
+
+def bench_batch(wl=None, log_prefix="batch"):
     """48 views of the 1M-triangle procedural scene (Hosek sky), resolution in
     {256^2, 512^2, 1280x720, 1920x1080} and spp in {1, 4, 16, 64}: one
     wavefront batch (after autotuning on a one-sample copy) against rendering
     the views one by one with the megakernel."""
     from noor_amd import scenes
     from noor_amd.renderer import Renderer
-    wl = scenes.procedural(width=256, height=256)
+    if wl is None:
+        wl = scenes.procedural(width=256, height=256)
     views = scenes.batch_views(48)
     samples = sum(c.w * c.h * spp for c, _, spp, _ in views)
     r = Renderer.from_workload(wl)
@@ -274,6 +420,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-textured", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C0, C1 and FP64 legs")
     ap.add_argument("--tri", default="watertight", choices=["watertight", "wald"],
                     help="triangle test (wald: the reference's projection test)")
     ap.add_argument("--textured-only", action="store_true", help="run only the texture-pipeline leg (profiling)")
@@ -377,27 +524,57 @@ def main():
     log("kernel times: " + ", ".join(f"{k} {v[0]}x {v[1]:.2f}ms" for k, v in kt.items()))
 
     ext_rays, shd_rays = st["extension_rays"], st["shadow_rays"]
-    bytes_kernel = {
+    # the CPU baseline (untimed by the steps): also the oracle's count of the
+    # reference's traversal events over the same rays
+    cpu = None
+    if not args.no_cpu:
+        wl.options.stats = 0
+        cb, cpu_film, rows, cpu_spp, cpu_cnt = cpu_baseline(wl, args.cpu_seconds)
+        cpu = (cb, cpu_film, rows, cpu_spp, cpu_cnt)
+    build_bytes = {
         "wave_extend": traversal_bytes(st, "", tri_test) + ext_rays * (4 + 32 + 24 + 4 + 16),
         "wave_shadow": traversal_bytes(st, "shadow_", tri_test) + shd_rays * (4 + 32 + 32 + 32),
     }
-    dom = max((k for k in kt if k in bytes_kernel), key=lambda k: kt[k][1])
+    dom = max((k for k in kt if k in build_bytes), key=lambda k: kt[k][1])
     launches, ms = kt[dom]
     avg_s = ms / 1e3 / launches
-    per_launch = bytes_kernel[dom] / launches
+    build_per_launch = build_bytes[dom] / launches
+    ref_per_launch = None
+    if cpu is not None:
+        c = cpu[4]
+        if dom == "wave_extend":
+            ref_per_launch = reference_bytes(c) / max(1, c.extension_rays) * ext_rays / launches
+        else:
+            ref_per_launch = reference_bytes(c, shadow=True) / max(1, c.shadow_rays) * shd_rays / launches
+    per_launch = ref_per_launch if ref_per_launch is not None else build_per_launch
     achieved = per_launch / avg_s / 1e9
-    traffic = None
-    if args.queue == 0 and args.block == 64:
-        traffic = MEASURED_HBM_BYTES.get((args.workload, wl.camera.w, wl.camera.h, spp, dom))
+    prof = measured_profile() if (args.queue == 0 and args.block == 64 and args.workload == "terrain") else None
+    kprof = prof.get("kernels", {}).get(dom) if prof else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": kprof["hbm_bytes_per_dispatch"] if kprof else None,
+                "bytes_per_launch": int(per_launch),
+                "bytes_layout": "reference (binary BVH, Wald records; oracle-counted events)"
+                if ref_per_launch is not None else "this build's wide BVH (no CPU sample to count the reference's)",
+                "achieved_build_layout": round(build_per_launch / avg_s / 1e9, 1),
+                "bytes_per_launch_build_layout": int(build_per_launch),
+                "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches,
+                "limiter": ({"valu_busy": kprof["valu_busy"], "lanes_active": kprof["lanes_active"],
+                             "what": "VALU issue with divergent lanes (hardware counters, see traffic source)"}
+                            if kprof else None),
+                "traffic_source": (os.path.relpath(PMC_PROFILE, ROOT) if kprof else None)}
     mega_roof = None
     if "megakernel" in kt:
         mb = traversal_bytes(st, "", tri_test) + traversal_bytes(st, "shadow_", tri_test)
         ms_m = kt["megakernel"][1] / kt["megakernel"][0]
-        mega_roof = {"achieved": round(mb / (ms_m / 1e3) / 1e9, 1), "avg_launch_ms": round(ms_m, 3),
-                     "bytes_per_launch": int(mb)}
+        mega_roof = {"achieved_build_layout": round(mb / (ms_m / 1e3) / 1e9, 1), "avg_launch_ms": round(ms_m, 3),
+                     "bytes_per_launch_build_layout": int(mb)}
+        if cpu is not None:
+            c = cpu[4]
+            rb = (reference_bytes(c) / max(1, c.extension_rays) * ext_rays +
+                  reference_bytes(c, shadow=True) / max(1, c.shadow_rays) * shd_rays)
+            mega_roof["achieved"] = round(rb / (ms_m / 1e3) / 1e9, 1)
+            mega_roof["bytes_per_launch"] = int(rb)
 
     wave_mrays = rays_per_step * args.steps / t_wave / 1e6
     out = {
@@ -431,10 +608,14 @@ def main():
         out["wavefront_vs_megakernel"] = round(wave_mrays / mega_mrays, 3)
         diff = np.abs(wave_film[..., :3].astype(np.float64) - mega_film[..., :3])
         out["image_mean_rel_err_wave_vs_mega"] = float(np.mean(diff / (np.abs(mega_film[..., :3]) + 1e-3)))
-    if not args.no_cpu:
-        wl.options.stats = 0
-        cb, cpu_film, rows, cpu_spp = cpu_baseline(wl, args.cpu_seconds)
+    if cpu is not None:
+        cb, cpu_film, rows, cpu_spp, cpu_cnt = cpu
         out["cpu_baseline"] = cb
+        out["reference_events_per_ray"] = {
+            "extension": {k: round(getattr(cpu_cnt, k) / max(1, cpu_cnt.extension_rays), 3)
+                          for k in ("node_visits", "tri_tests", "tri_second_fetch", "instance_entries")},
+            "shadow": {k: round(getattr(cpu_cnt, "shd_" + k) / max(1, cpu_cnt.shadow_rays), 3)
+                       for k in ("node_visits", "tri_tests", "tri_second_fetch", "instance_entries")}}
         # the CPU sample against the GPU film of the same rows and samples
         if cpu_spp != spp:
             r.render(spp=cpu_spp, frame=1)
@@ -450,8 +631,20 @@ def main():
     r.close()
     del r
     out["scene_build"] = build
+    c1 = None
+    if not args.no_configs:
+        from noor_amd import scenes as sc
+        cfg = {}
+        c0 = sc.cornell()
+        cfg["C0"] = bench_config(c0, "C0 Cornell + displaced sphere", 0, 5, cpu_full=not args.no_cpu)
+        cfg["C0"]["fp64_oracle"] = bench_fp64_rmse(c0)
+        del c0
+        c1 = sc.procedural()
+        cfg["C1"] = bench_config(c1, "C1 procedural, Hosek sky", 0 if args.no_cpu else 10.0, 2)
+        out["configs"] = cfg
     if not args.no_batch:
-        out["batch"] = bench_batch()
+        out["batch"] = bench_batch(c1)
+    del c1
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint()
     if not args.no_textured:

@@ -338,8 +338,10 @@ typedef struct noor_kernel_time { char name[48]; uint64_t launches; double total
 noor_status noor_set_timing(noor_ctx* ctx, int enable);
 noor_status noor_get_kernel_times(noor_ctx* ctx, noor_kernel_time* out, uint32_t max, uint32_t* count);
 noor_status noor_reset_kernel_times(noor_ctx* ctx);
-/* Wavefront tunables: queue capacity in paths (0 = automatic) and lanes per
- * block of the traversal kernels (64, 128 or 256). */
+/* Wavefront tunables: queue capacity in paths (0 = automatic: the items of
+ * the render, at most 2^25, about 8 GB of path state, halved down to 2^22
+ * when the device cannot hold it) and lanes per block of the traversal
+ * kernels (64, 128 or 256). */
 noor_status noor_set_wavefront_params(noor_ctx* ctx, uint32_t queue_capacity, int32_t block);
 
 /* Traversal stack entries per lane kept in LDS (the rest of the exact
@@ -364,7 +366,8 @@ noor_status noor_read_batch_film(noor_ctx* ctx, uint32_t view, float* rgba, size
 noor_status noor_batch_film_device_ptr(noor_ctx* ctx, uint32_t view, void** ptr);
 
 /* Wavefront autotuner: times noor_render_batch(views) for every setting of
- * block {64, 128, 256} x queue capacity {2^20 .. 2^23} x LDS stack entries
+ * block {64, 128, 256} x queue capacity {2^20 .. 2^25, up to the first that
+ * holds every item of the batch} x LDS stack entries
  * {16, 24, 32} (clamped to what the BVH needs), keeps the fastest on the
  * context and returns the measurements (at most `max`).  The batch films are
  * overwritten. */

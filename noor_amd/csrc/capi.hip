@@ -763,7 +763,14 @@ noor_status noor_autotune(noor_ctx* ctx, const noor_view* views, uint32_t n, noo
     if (st != NOOR_OK) return st;
     if ((st = batch_films(c, views, n)) != NOOR_OK) return st;
     const int blocks[3] = {64, 128, 256};
-    const uint32_t caps[4] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    // pools of 2^20 .. 2^25 paths; larger ones than the batch's items add nothing
+    uint64_t items = 0;
+    for (uint32_t i = 0; i < n; ++i) items += (uint64_t)views[i].camera.w * views[i].camera.h * views[i].spp;
+    std::vector<uint32_t> caps;
+    for (uint32_t cap = 1u << 20; cap <= (1u << 25); cap <<= 1) {
+        caps.push_back(cap);
+        if (cap >= items) break;
+    }
     std::vector<int> ks;
     for (int k : {16, 24, 32}) {
         const int kk = std::min(k, c->S.stack_depth);

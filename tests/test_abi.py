@@ -202,3 +202,21 @@ def test_wide_bvh_check_rejects_corrupt_payload(small):
     keep = np.ascontiguousarray(a, np.uint32)
     v.bvh_min_right_start = keep.ctypes.data_as(C.POINTER(abi.uint4))
     assert lib().noor_wide_bvh_check(C.byref(v), None, None) == abi.ERR_INVALID_PAYLOAD
+
+
+def test_pod_offsets_match_reference_layouts(tmp_path):
+    """tests/abi_offsets.c asserts every field offset of the camera, spec,
+    material and Hosek PODs against the reference's CUDA declarations; a
+    planted offset (the spec's sky type packed after the shorts, as a 2-byte
+    aligned enum would sit) must fail the same check."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=c11", "-I", inc, os.path.join(ROOT, "tests", "abi_offsets.c")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    bad = tmp_path / "bad.c"
+    bad.write_text('#include <stddef.h>\n#include "noor_hip.h"\n'
+                   '_Static_assert(offsetof(noor_spec, skydome_type) == 54, "planted");\n')
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=c11", "-I", inc, str(bad)], capture_output=True, text=True)
+    assert r.returncode != 0 and "planted" in r.stderr
+

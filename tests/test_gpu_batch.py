@@ -86,10 +86,13 @@ def test_autotune_picks_a_measured_setting(wl):
     cams = _cams()
     r = Renderer.from_workload(wl)
     res = r.autotune([(c, 1, 1, 0) for c in cams])
-    assert len(res) >= 12
+    # pools from 2^20 up to the first that holds every item (these few views:
+    # 2^20 only), three block sizes, the LDS stack splits the BVH allows
+    assert len(res) >= 9
     assert all(ms > 0 for *_, ms in res)
     blocks = {b for _, b, _, _ in res}
     assert blocks == {64, 128, 256}
+    assert {q for q, *_ in res} == {1 << 20}
     # the tuned context still renders the batch identically
     r.render_batch([(c, 1, 1, 0) for c in cams])
     m = Renderer(wl.payload, cams[0], wl.spec, wl.options, wl.hosek)
