@@ -934,8 +934,10 @@ noor_status noor_get_stats(noor_ctx* ctx, noor_stats* out) {
     HIP_TRY(c, hipMemcpyAsync(&d, c->d_stats, sizeof(d), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::memset(out, 0, sizeof(*out));
-    out->extension_rays = d.extension_rays;
-    out->shadow_rays = d.shadow_rays;
+    for (const auto& sl : d.slot) {
+        out->extension_rays += sl.extension_rays;
+        out->shadow_rays += sl.shadow_rays;
+    }
     out->node_visits = d.ext[0];
     out->tri_tests = d.ext[1];
     out->tri_second_fetch = d.ext[2];

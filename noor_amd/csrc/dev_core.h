@@ -242,14 +242,19 @@ struct Ray {
     float Sx, Sy, Sz;
 };
 
-__device__ __forceinline__ void ray_update(Ray& r) {
+// Box-test terms of a ray: reciprocal direction and octant flags.
+__device__ __forceinline__ void ray_update_box(Ray& r) {
     r.inv = f3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
     r.pn = (r.d.x > 0.0f ? 0u : 1u) | (r.d.y > 0.0f ? 0u : 2u) | (r.d.z > 0.0f ? 0u : 4u);
     // the fast slab form needs finite reciprocals whose products with a
     // node's step cannot overflow (steps stay far below 2^40)
     const float mi = fmaxf(fmaxf(fabsf(r.inv.x), fabsf(r.inv.y)), fabsf(r.inv.z));
     if (mi <= 0x1p80f) r.pn |= 8u;
-    // Woop, Benthin & Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013
+}
+
+// Triangle-test terms: Woop, Benthin & Wald, "Watertight Ray/Triangle
+// Intersection", JCGT 2(1), 2013 (axis permutation and shear).  Needs inv.
+__device__ __forceinline__ void ray_update_tri(Ray& r) {
     const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
     int kz = 0;
     float m = ax;
@@ -273,7 +278,12 @@ __device__ __forceinline__ void ray_update(Ray& r) {
     const float dz = c3(r.d, kz);
     r.Sx = c3(r.d, kx) / dz;
     r.Sy = c3(r.d, ky) / dz;
-    r.Sz = 1.0f / dz;
+    r.Sz = c3(r.inv, kz);  // 1 / dz, already rounded once in inv
+}
+
+__device__ __forceinline__ void ray_update(Ray& r) {
+    ray_update_box(r);
+    ray_update_tri(r);
 }
 __device__ __forceinline__ Ray mkray(f3 o, f3 d, float tmax) {
     Ray r;
@@ -671,16 +681,41 @@ __device__ __forceinline__ void closest_begin(const DevScene& S, const Ray& r, C
     t.h.u = t.h.v = 0.0f;
 }
 
+// One step of a traversal visits one inner node, tests ONE triangle of a leaf
+// (a leaf with triangles left stays the current reference, its start advanced
+// and its count decremented), enters one instance or tests one light.  Lanes
+// of a wave step in lockstep, so a wave pays for the union of the kinds its
+// lanes visit in a step: testing a whole leaf per step would make every step
+// with a leaf lane cost as many triangle tests as the wave's largest leaf.
+// The ray of the level is rebuilt in one place for both transitions that need
+// it: entering an instance (object space) and popping back to the top level
+// (world space); a pop straight into another instance builds only the
+// instance's ray, and the top level's ray gets only its box-test terms.
+__device__ __forceinline__ uint32_t leaf_next(uint32_t cur) {
+    return ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) ? cur + 1u - (1u << REF_LEAF_COUNT_SHIFT) : REF_EMPTY;
+}
+
 template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t, const TravStack& st, Counters& c,
                                               bool& overflow, int max_steps) {
     for (int step = 0; step < max_steps; ++step) {
         const uint32_t cur = t.cur;
-        if ((cur & REF_TLAS) && !t.world) {
-            t.lray = mkray(t.o, t.d, t.lray.tmax);
-            t.world = true;
-        }
         const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
+        if (kind == REF_INST || ((cur & REF_TLAS) && !t.world)) {
+            f3 o = t.o, d = t.d;
+            if (kind == REF_INST) {
+                t.ins = cur & REF_INDEX_MASK;
+                if (STATS) c.inst++;
+                const Xf T = xf_load(S.xf + 9 * (size_t)t.ins);
+                o = xf_pt(T, t.o);
+                d = xf_vec(T, t.d);
+            }
+            t.lray.o = o;
+            t.lray.d = d;
+            ray_update_box(t.lray);
+            if (kind == REF_INST) ray_update_tri(t.lray);  // the top level holds no triangles
+            t.world = kind != REF_INST;
+        }
         if (kind == REF_INNER) {
             if (STATS) c.nodes++;
             const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
@@ -708,11 +743,9 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
                 continue;
             }
         } else if (kind == REF_LEAF) {
-            const uint32_t start = cur & REF_LEAF_START_MASK;
-            const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
-            for (uint32_t ti = start; ti < end; ++ti) {
-                float tt, u, v;
-                if (!tri_test<STATS, ALPHA>(S, t.lray, ti, tt, u, v, c)) continue;
+            const uint32_t ti = cur & REF_LEAF_START_MASK;
+            float tt, u, v;
+            if (tri_test<STATS, ALPHA>(S, t.lray, ti, tt, u, v, c)) {
                 const uint32_t key = ldg(S.prim_ids + ti);
                 bool take = false;
                 if (tt < t.lray.tmax) take = true;
@@ -732,12 +765,9 @@ __device__ __forceinline__ bool closest_steps(const DevScene& S, ClosestTrav& t,
                     t.h.kind = 1;
                 }
             }
+            t.cur = leaf_next(cur);
+            if (t.cur != REF_EMPTY) continue;
         } else if (kind == REF_INST) {
-            t.ins = cur & REF_INDEX_MASK;
-            if (STATS) c.inst++;
-            const Xf T = xf_load(S.xf + 9 * (size_t)t.ins);
-            t.lray = mkray(xf_pt(T, t.o), xf_vec(T, t.d), t.lray.tmax);
-            t.world = false;
             t.cur = ldg(S.inst_root + t.ins);
             if (t.cur != REF_EMPTY) continue;
         } else {
@@ -793,16 +823,27 @@ __device__ __forceinline__ void any_begin(const DevScene& S, const Ray& r, AnyTr
     t.occluded = false;
 }
 
+// Steps as in closest_steps (one node, triangle, instance or light per step).
 template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const TravStack& st, Counters& c,
                                           bool& overflow, int max_steps) {
     for (int step = 0; step < max_steps; ++step) {
         const uint32_t cur = t.cur;
-        if ((cur & REF_TLAS) && !t.world) {
-            t.lray = mkray(t.o, t.d, t.lray.tmax);
-            t.world = true;
-        }
         const uint32_t kind = (cur >> REF_KIND_SHIFT) & 3u;
+        if (kind == REF_INST || ((cur & REF_TLAS) && !t.world)) {
+            f3 o = t.o, d = t.d;
+            if (kind == REF_INST) {
+                if (STATS) c.inst++;
+                const Xf T = xf_load(S.xf + 9 * (size_t)(cur & REF_INDEX_MASK));
+                o = xf_pt(T, t.o);
+                d = xf_vec(T, t.d);
+            }
+            t.lray.o = o;
+            t.lray.d = d;
+            ray_update_box(t.lray);
+            if (kind == REF_INST) ray_update_tri(t.lray);  // the top level holds no triangles
+            t.world = kind != REF_INST;
+        }
         if (kind == REF_INNER) {
             if (STATS) c.nodes++;
             const WideNode n = load_wnode(S, cur & REF_INDEX_MASK);
@@ -824,22 +865,15 @@ __device__ __forceinline__ bool any_steps(const DevScene& S, AnyTrav& t, const T
                 continue;
             }
         } else if (kind == REF_LEAF) {
-            const uint32_t start = cur & REF_LEAF_START_MASK;
-            const uint32_t end = start + ((cur >> REF_LEAF_COUNT_SHIFT) & 7u) + 1u;
-            for (uint32_t ti = start; ti < end; ++ti) {
-                float tt, u, v;
-                if (tri_test<STATS, ALPHA>(S, t.lray, ti, tt, u, v, c)) {
-                    t.occluded = true;
-                    return true;
-                }
+            float tt, u, v;
+            if (tri_test<STATS, ALPHA>(S, t.lray, cur & REF_LEAF_START_MASK, tt, u, v, c)) {
+                t.occluded = true;
+                return true;
             }
+            t.cur = leaf_next(cur);
+            if (t.cur != REF_EMPTY) continue;
         } else if (kind == REF_INST) {
-            const uint32_t ins = cur & REF_INDEX_MASK;
-            if (STATS) c.inst++;
-            const Xf T = xf_load(S.xf + 9 * (size_t)ins);
-            t.lray = mkray(xf_pt(T, t.o), xf_vec(T, t.d), t.lray.tmax);
-            t.world = false;
-            t.cur = ldg(S.inst_root + ins);
+            t.cur = ldg(S.inst_root + (cur & REF_INDEX_MASK));
             if (t.cur != REF_EMPTY) continue;
         } else {
             Ray tr = t.lray;

@@ -29,8 +29,9 @@ __device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext
                                             const Counters& ce, const Counters& cs, bool overflow) {
     const unsigned long long e = wave_sum(ext), s = wave_sum(shd), ov = wave_sum(overflow ? 1ull : 0ull);
     if ((threadIdx.x & 63) == 0) {
-        if (e) atomicAdd(&st->extension_rays, e);
-        if (s) atomicAdd(&st->shadow_rays, s);
+        DevStats::Slot& sl = st->slot[blockIdx.x % kStatSlots];  // one wave per block
+        if (e) atomicAdd(&sl.extension_rays, e);
+        if (s) atomicAdd(&sl.shadow_rays, s);
         if (ov) atomicAdd(&st->overflow, ov);
     }
     flush_counters(st->ext, ce);

@@ -173,6 +173,11 @@ __device__ __forceinline__ void block_append3(uint32_t* c0, bool f0, uint32_t* c
     s2 = cnt[2][wave] + (uint32_t)__popcll(m2 & below);
 }
 
+// Counter slot of this wave for the ray counts (spread over kStatSlots).
+__device__ __forceinline__ uint32_t stat_slot() {
+    return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kStatSlots;
+}
+
 __device__ __forceinline__ void flush4(unsigned long long* dst, const Counters& c) {
     unsigned long long v[4] = {c.nodes, c.tris, c.tri2, c.inst};
     for (int i = 0; i < 4; ++i)
@@ -375,14 +380,14 @@ __global__ void k_extend(const DevScene* __restrict__ Sp, WavePool B, int cur, D
     }
     unsigned long long e = ext;
     for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);  // 64-bit wave sum
-    if (lane_id() == 0 && e) atomicAdd(&stats->extension_rays, e);
+    if (lane_id() == 0 && e) atomicAdd(&stats->slot[stat_slot()].extension_rays, e);
     if (STATS) flush4(stats->ext, cnt);
     const unsigned long long ov = __ballot(overflow);
     if (lane_id() == 0 && ov) atomicAdd(&stats->overflow, 1ull);
 }
 
 // Material-class binning of the extension queue, in queue order: one LDS
-// histogram and one atomic per class per block.
+// histogram and one atomic per class per block (blocks of up to 16 waves).
 __global__ void k_bin(WavePool B, int cur) {
     WaveCtl* ctl = B.ctl;
     const uint32_t count = ctl->ray_count[cur] + ctl->new_rays[cur];
@@ -568,7 +573,7 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WavePool B, DevStats* 
     }
     unsigned long long v = shd;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane_id() == 0 && v) atomicAdd(&stats->shadow_rays, v);
+    if (lane_id() == 0 && v) atomicAdd(&stats->slot[stat_slot()].shadow_rays, v);
     if (STATS) flush4(stats->shd, cnt);
     const unsigned long long ov = __ballot(overflow);
     if (lane_id() == 0 && ov) atomicAdd(&stats->overflow, 1ull);
@@ -676,6 +681,10 @@ void wave_free(Context& c) {
 static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
     const uint32_t cap = B.P.capacity;
     const dim3 g1((cap + 255) / 256), b1(256);
+    // k_bin takes one atomic per class per block: with returning atomics on
+    // one address completing about one per 11 ns (measured), 256-lane blocks
+    // left it bound by them; 1024-lane blocks cut its time to a third
+    const dim3 gb((cap + 1023) / 1024), bb(1024);
     const int blk = B.P.block;
     const size_t lds = (size_t)blk * B.P.stack_lds * sizeof(uint32_t);
     const dim3 gp(B.persist_waves / (blk / 64)), bp(blk);
@@ -698,7 +707,7 @@ static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
         hipLaunchKernelGGL(extend, gp, bp, lds, c.stream, c.d_scene, B.P, cur, c.d_stats);
         T.end(c.stream);
         T.begin(c.stream, "wave_bin");
-        hipLaunchKernelGGL(k_bin, g1, b1, 0, c.stream, B.P, cur);
+        hipLaunchKernelGGL(k_bin, gb, bb, 0, c.stream, B.P, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_shade");
         hipLaunchKernelGGL(shade, g1, b1, 0, c.stream, c.d_scene, B.P, cur);
