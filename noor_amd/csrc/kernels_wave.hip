@@ -91,8 +91,6 @@ struct WavePool {
     float4* beta = nullptr;    // beta.xyz, w = bounce | specular << 16
     uint64_t* rng = nullptr;
     uint64_t* rng_inc = nullptr;
-    float4* ro = nullptr;
-    float4* rd = nullptr;
     // ray differentials of textured scenes: auxiliary origins (w of dox = has)
     // and directions
     float4* dox = nullptr;
@@ -101,12 +99,12 @@ struct WavePool {
     float4* ddy = nullptr;
     bool tex = false;
     float4* hit = nullptr;     // t, u, v, prim
-    uint2* hit2 = nullptr;     // inst, kind
+    uint2* hit2 = nullptr;     // inst, kind | extension-queue position << 2
     uint32_t* cls = nullptr;   // material class of the hit (k_extend -> k_bin)
     float4* nee_a = nullptr;   // candidate.xyz
     float4* nee_beta = nullptr;
-    float4* sh_o = nullptr;    // shadow ray origin, tmax
-    float4* sh_d = nullptr;
+    float4* sh_o = nullptr;    // by shadow-queue position: shadow ray origin, tmax
+    float4* sh_d = nullptr;    // by shadow-queue position: direction
     // MIS: the BSDF-sampled candidate and its shadow ray; w of nee_a / nee_b
     // flags which of the two candidates need their occlusion test
     float4* nee_b = nullptr;
@@ -114,6 +112,11 @@ struct WavePool {
     float4* sh2_d = nullptr;
     bool mis = false;
     uint32_t* q_ray[2] = {nullptr, nullptr};
+    // the rays of the extension queues, by queue position (origin, direction):
+    // a persistent kernel's refill reads them with its queue entries instead
+    // of through the path slot, and k_shade's appends write them in order
+    float4* qo[2] = {nullptr, nullptr};
+    float4* qd[2] = {nullptr, nullptr};
     uint32_t* q_free[2] = {nullptr, nullptr};
     uint32_t* q_class = nullptr;  // kClasses * capacity
     uint32_t* q_shadow = nullptr;
@@ -323,9 +326,9 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
     B.view[p] = sg.view;
     B.rng_inc[p] = ps.rng.inc;
     store_state(B, p, ps);
-    B.ro[p] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-    B.rd[p] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
     B.q_ray[cur][c0 + t] = p;
+    B.qo[cur][c0 + t] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+    B.qd[cur][c0 + t] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
 }
 
 // Refill of a persistent wave: when at least kRefill lanes are idle (or all
@@ -362,13 +365,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     Counters cnt{0, 0, 0, 0};
     bool overflow = false, have = false, exhausted = false;
     uint32_t ext = 0;  // rays this lane started
-    uint32_t p = 0;
+    uint32_t p = 0, qi = 0;
     ClosestTrav t;
     for (;;) {
         const uint32_t idx = refill(&ctl->work_extend, count, have, exhausted);
         if (idx != 0xFFFFFFFFu) {
             p = q[idx];
-            const float4 o = B.ro[p], d = B.rd[p];
+            qi = idx;
+            const float4 o = B.qo[cur][idx], d = B.qd[cur][idx];
             closest_begin(S, mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, K_INF), t);
             have = true;
             ++ext;
@@ -376,7 +380,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         if (__ballot(have) == 0ull) break;
         if (have && closest_steps<STATS, ALPHA>(S, t, st, cnt, overflow, B.steps)) {
             B.hit[p] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.prim));
-            B.hit2[p] = make_uint2(t.h.inst, t.hit ? t.h.kind : 0u);
+            B.hit2[p] = make_uint2(t.h.inst, (t.hit ? t.h.kind : 0u) | (qi << 2));
             B.cls[p] = (uint32_t)material_class(S, t.hit, t.h);
             have = false;
         }
@@ -458,23 +462,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
     const int nxt = cur ^ 1;
     bool cont = false, shadow = false, ended = false;
     uint32_t p = 0;
+    float4 sho, shd, nro, nrd;  // the shadow ray and the next ray, written at their queue positions
     if (cls >= 0) {
         p = B.q_class[(size_t)cls * B.capacity + local];
         PathState ps;
         load_state(B, p, ps);
-        const float4 o = B.ro[p], d = B.rd[p];
-        Ray ray = mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, K_INF);
         const float4 hh = B.hit[p];
         const uint2 h2 = B.hit2[p];
+        const float4 o = B.qo[cur][h2.y >> 2], d = B.qd[cur][h2.y >> 2];
+        Ray ray = mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, K_INF);
         Hit h;
         h.t = hh.x;
         h.u = hh.y;
         h.v = hh.z;
         h.prim = __float_as_uint(hh.w);
         h.inst = h2.x;
-        h.kind = h2.y;
+        h.kind = h2.y & 3u;
         h.key = 0;
-        const bool hit = h2.y != 0;
+        const bool hit = h.kind != 0;
         if (hit) ray.tmax = h.t;
         Nee nee;
         RayDiff rdf;
@@ -487,8 +492,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
             shadow = true;
             B.nee_a[p] = make_float4(nee.a.x, nee.a.y, nee.a.z, __int_as_float(nee.na));
             B.nee_beta[p] = make_float4(nee.beta.x, nee.beta.y, nee.beta.z, 0.0f);
-            B.sh_o[p] = make_float4(nee.ra.o.x, nee.ra.o.y, nee.ra.o.z, nee.ra.tmax);
-            B.sh_d[p] = make_float4(nee.ra.d.x, nee.ra.d.y, nee.ra.d.z, 0.0f);
+            sho = make_float4(nee.ra.o.x, nee.ra.o.y, nee.ra.o.z, nee.ra.tmax);
+            shd = make_float4(nee.ra.d.x, nee.ra.d.y, nee.ra.d.z, 0.0f);
             if (nee.nb) {  // only with MIS
                 B.nee_b[p] = make_float4(nee.b.x, nee.b.y, nee.b.z, 1.0f);
                 B.sh2_o[p] = make_float4(nee.rb.o.x, nee.rb.o.y, nee.rb.o.z, nee.rb.tmax);
@@ -501,16 +506,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
         }
         if (cont) {
             ps.bounce++;
-            B.ro[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, 0.0f);
-            B.rd[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
+            nro = make_float4(ps.o.x, ps.o.y, ps.o.z, 0.0f);
+            nrd = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
             if (TEX) store_diff(B, p, rdf);
         }
         store_state(B, p, ps);
     }
     uint32_t s1, s2, s3;
     block_append3(&ctl->shadow_count, shadow, &ctl->ray_count[nxt], cont, &ctl->free_count[nxt], ended, s1, s2, s3);
-    if (shadow) B.q_shadow[s1] = p;
-    if (cont) B.q_ray[nxt][s2] = p;
+    if (shadow) {
+        B.q_shadow[s1] = p;
+        B.sh_o[s1] = sho;
+        B.sh_d[s1] = shd;
+    }
+    if (cont) {
+        B.q_ray[nxt][s2] = p;
+        B.qo[nxt][s2] = nro;
+        B.qd[nxt][s2] = nrd;
+    }
     if (ended) B.q_free[nxt][s3] = p;
 }
 
@@ -537,7 +550,7 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WavePool B, DevStats* 
             p = B.q_shadow[idx];
             occ_a = false;
             second = MIS && __float_as_int(B.nee_a[p].w) == 0;  // no light-sampled ray: the BSDF-sampled one only
-            const float4 o = second ? B.sh2_o[p] : B.sh_o[p], d = second ? B.sh2_d[p] : B.sh_d[p];
+            const float4 o = second ? B.sh2_o[p] : B.sh_o[idx], d = second ? B.sh2_d[p] : B.sh_d[idx];
             any_begin(S, mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, o.w), t);
             have = true;
             ++shd;
@@ -629,8 +642,6 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
     WA(P.beta, cap);
     WA(P.rng, cap);
     WA(P.rng_inc, cap);
-    WA(P.ro, cap);
-    WA(P.rd, cap);
     if (tex) {
         WA(P.dox, cap);
         WA(P.doy, cap);
@@ -653,6 +664,10 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
     P.mis = mis;
     WA(P.q_ray[0], cap);
     WA(P.q_ray[1], cap);
+    WA(P.qo[0], cap);
+    WA(P.qo[1], cap);
+    WA(P.qd[0], cap);
+    WA(P.qd[1], cap);
     WA(P.q_free[0], cap);
     WA(P.q_free[1], cap);
     WA(P.q_class, (size_t)kClasses * cap);
