@@ -109,6 +109,32 @@ def reference_bytes(cnt, shadow=False):
             16 * getattr(cnt, p + "tri_second_fetch") + 48 * getattr(cnt, p + "instance_entries"))
 
 
+def shade_roofline(st, paths, kt, prof):
+    """k_shade's algorithmic bytes per launch and its achieved rate.  Per
+    shaded path: path state read (class-queue entry 4, L 16, beta and bounce
+    16, RNG 16, hit record 24, ray 32) and written (L, beta, RNG: 40); per
+    continuing ray the next ray and its queue entry (36); per shadow task the
+    NEE candidate, shadow ray and queue entry (68); per ended path its
+    free-queue entry (4); per hit the shading fetch (attributes 16, three
+    vertices, normals and uvs 120, object-to-world and normal rows 96: 232)
+    and the material record (128); per miss the environment lookup (64)."""
+    if "wave_shade" not in kt:
+        return None
+    ext, shd, hits = st["extension_rays"], st["shadow_rays"], st["extension_hits"]
+    cont = ext - paths  # every extension ray after a path's first
+    total = (ext * (108 + 40) + cont * 36 + shd * 68 + paths * 4 + hits * (232 + 128) + (ext - hits) * 64)
+    launches, ms = kt["wave_shade"]
+    per_launch = total / launches
+    avg_s = ms / 1e3 / launches
+    kp = (prof or {}).get("kernels", {}).get("wave_shade")
+    return {"bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches,
+            "achieved": round(per_launch / avg_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(per_launch / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_path": round(total / max(1, ext), 1),
+            "traffic": kp["hbm_bytes_per_dispatch"] if kp else None,
+            "traffic_over_algorithmic": round(kp["hbm_bytes_per_dispatch"] / per_launch, 3) if kp else None}
+
+
 def cpu_baseline(wl, target_s):
     from oracle.binding import OracleScene
     o = OracleScene.from_workload(wl)
@@ -531,8 +557,11 @@ def main():
         wl.options.stats = 0
         cb, cpu_film, rows, cpu_spp, cpu_cnt = cpu_baseline(wl, args.cpu_seconds)
         cpu = (cb, cpu_film, rows, cpu_spp, cpu_cnt)
+    # per ray beside the traversal: extension: queue entry 4, ray 32, hit record
+    # 24, material class 4; shadow: queue entry 4, ray 32, NEE candidate 32,
+    # radiance read and write 32
     build_bytes = {
-        "wave_extend": traversal_bytes(st, "", tri_test) + ext_rays * (4 + 32 + 24 + 4 + 16),
+        "wave_extend": traversal_bytes(st, "", tri_test) + ext_rays * (4 + 32 + 24 + 4),
         "wave_shadow": traversal_bytes(st, "shadow_", tri_test) + shd_rays * (4 + 32 + 32 + 32),
     }
     dom = max((k for k in kt if k in build_bytes), key=lambda k: kt[k][1])
@@ -563,6 +592,7 @@ def main():
                              "what": "VALU issue with divergent lanes (hardware counters, see traffic source)"}
                             if kprof else None),
                 "traffic_source": (os.path.relpath(PMC_PROFILE, ROOT) if kprof else None)}
+    shade = shade_roofline(st, pixels * spp, kt, prof)
     mega_roof = None
     if "megakernel" in kt:
         mb = traversal_bytes(st, "", tri_test) + traversal_bytes(st, "shadow_", tri_test)
@@ -600,6 +630,14 @@ def main():
         "wavefront": {"mrays_per_s": round(wave_mrays, 2), "ms_per_step": round(t_wave / args.steps * 1e3, 2)},
         "roofline": roofline,
         "kernel_ms": {k: round(v[1] / v[0], 4) for k, v in kt.items()},
+        "shade_roofline": shade,
+        "events_per_ray": {
+            "extension": {k: round(st[k] / max(1, ext_rays), 3)
+                          for k in ("node_visits", "tri_tests", "tri_second_fetch", "instance_entries")},
+            "shadow": {k: round(st["shadow_" + k] / max(1, shd_rays), 3)
+                       for k in ("node_visits", "tri_tests", "tri_second_fetch", "instance_entries")},
+            "extension_hit_fraction": round(st["extension_hits"] / max(1, ext_rays), 4),
+            "layout": "this build's four-wide BVH (device counters of the counting pass)"},
     }
     if t_mega is not None:
         mega_mrays = rays_per_step * args.steps / t_mega / 1e6

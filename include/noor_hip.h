@@ -36,7 +36,8 @@
 extern "C" {
 #endif
 
-#define NOOR_ABI_VERSION 1
+/* 2: noor_stats gained extension_hits */
+#define NOOR_ABI_VERSION 2
 
 typedef enum noor_status {
     NOOR_OK = 0,
@@ -269,6 +270,8 @@ typedef struct noor_stats {
     uint64_t node_visits, tri_tests, tri_second_fetch, instance_entries;
     /* traversal events of any-hit (shadow) queries */
     uint64_t shadow_node_visits, shadow_tri_tests, shadow_tri_second_fetch, shadow_instance_entries;
+    /* extension rays that hit a triangle or an area light (the rest escape) */
+    uint64_t extension_hits;
 } noor_stats;
 
 /* ---- hit queries (test hook; noor/cuda/bvh.cuh:142-234) ------------------ */

@@ -946,6 +946,7 @@ noor_status noor_get_stats(noor_ctx* ctx, noor_stats* out) {
     out->shadow_tri_tests = d.shd[1];
     out->shadow_tri_second_fetch = d.shd[2];
     out->shadow_instance_entries = d.shd[3];
+    out->extension_hits = d.ext_hits;
     if (d.overflow) return fail(c, NOOR_ERR_STACK_DEPTH, "traversal stack overflow");
     return NOOR_OK;
 }

@@ -25,6 +25,7 @@ struct DevStats {
     unsigned long long ext[4];   // nodes, triangle tests, second Wald fetches, instance entries
     unsigned long long shd[4];
     unsigned long long overflow;
+    unsigned long long ext_hits;  // closest-hit queries that hit (counted with STATS)
 };
 
 struct WaveBuffers;  // wavefront queues (kernels_wave.hip)

@@ -25,14 +25,18 @@ __device__ __forceinline__ void flush_counters(unsigned long long* dst, const Co
             if (v[i]) atomicAdd(&dst[i], v[i]);
 }
 
+// `hits` (closest-hit queries that hit) is counted with STATS only, else 0.
 __device__ __forceinline__ void flush_stats(DevStats* st, unsigned long long ext, unsigned long long shd,
-                                            const Counters& ce, const Counters& cs, bool overflow) {
+                                            unsigned long long hits, const Counters& ce, const Counters& cs,
+                                            bool overflow) {
     const unsigned long long e = wave_sum(ext), s = wave_sum(shd), ov = wave_sum(overflow ? 1ull : 0ull);
+    const unsigned long long nh = wave_sum(hits);
     if ((threadIdx.x & 63) == 0) {
         DevStats::Slot& sl = st->slot[blockIdx.x % kStatSlots];  // one wave per block
         if (e) atomicAdd(&sl.extension_rays, e);
         if (s) atomicAdd(&sl.shadow_rays, s);
         if (ov) atomicAdd(&st->overflow, ov);
+        if (nh) atomicAdd(&st->ext_hits, nh);
     }
     flush_counters(st->ext, ce);
     flush_counters(st->shd, cs);
@@ -55,7 +59,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const uint32_t tiles_x = (W + 7) / 8, tiles = tiles_x * ((H + 7) / 8);
     const TravStack st = make_stack(S, smem);
     Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
-    unsigned long long ext = 0, shd = 0;
+    unsigned long long ext = 0, shd = 0, nhit = 0;
     bool overflow = false;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint32_t x = (tile % tiles_x) * 8 + (lane & 7);
@@ -83,6 +87,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 Hit h;
                 ++ext;
                 const bool hit = bvh_closest<STATS, TEX>(S, ray, h, st, cnt, overflow);
+                if (STATS && hit) ++nhit;
                 Nee nee;
                 const bool cont =
                     shade_bounce<TEX, DIRECT>(S, ps, ray, hit, h, nee, center ? &lookat : nullptr, nullptr, &rd);
@@ -113,7 +118,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         else film[gid] = acc;
         if (center && lookat_out) *lookat_out = lookat;
     }
-    flush_stats(stats, ext, shd, cnt, cns, overflow);
+    flush_stats(stats, ext, shd, nhit, cnt, cns, overflow);
 }
 
 hipError_t launch_megakernel(const Context& c, uint32_t frame0, uint32_t spp, float4* direct, float4* direct_lo) {
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, c
     const TravStack st = make_stack(S, smem, gstack, glanes);
     Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
     bool overflow = false;
-    unsigned long long ext = 0, shd = 0;
+    unsigned long long ext = 0, shd = 0, nhit = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const noor_ray r = rays[i];
         Ray ray = mkray(f3{r.o.x, r.o.y, r.o.z}, f3{r.d.x, r.d.y, r.d.z}, r.tmax);
@@ -173,6 +178,7 @@ __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, c
             ++ext;
             Hit h;
             if (bvh_closest<STATS, ALPHA>(S, ray, h, st, cnt, overflow)) {
+                if (STATS) ++nhit;
                 out.t = h.t;
                 out.u = h.u;
                 out.v = h.v;
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(64) void k_trace(const DevScene* __restrict__ Sp, c
         }
         hits[i] = out;
     }
-    flush_stats(stats, ext, shd, cnt, cns, overflow);
+    flush_stats(stats, ext, shd, nhit, cnt, cns, overflow);
 }
 
 hipError_t launch_trace(const Context& c, const noor_ray* d_rays, uint32_t n, int any_hit, noor_hit* d_hits,

@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint32_t* q = B.q_ray[cur];
     Counters cnt{0, 0, 0, 0};
     bool overflow = false, have = false, exhausted = false;
-    uint32_t ext = 0;  // rays this lane started
+    uint32_t ext = 0, nhit = 0;  // rays this lane started, and (STATS) how many hit
     uint32_t p = 0, qi = 0;
     ClosestTrav t;
     for (;;) {
@@ -382,13 +382,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             B.hit[p] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.prim));
             B.hit2[p] = make_uint2(t.h.inst, (t.hit ? t.h.kind : 0u) | (qi << 2));
             B.cls[p] = (uint32_t)material_class(S, t.hit, t.h);
+            if (STATS && t.hit) ++nhit;
             have = false;
         }
     }
     unsigned long long e = ext;
     for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);  // 64-bit wave sum
     if (lane_id() == 0 && e) atomicAdd(&stats->slot[stat_slot()].extension_rays, e);
-    if (STATS) flush4(stats->ext, cnt);
+    if (STATS) {
+        flush4(stats->ext, cnt);
+        unsigned long long h = nhit;
+        for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off, 64);
+        if (lane_id() == 0 && h) atomicAdd(&stats->ext_hits, h);
+    }
     const unsigned long long ov = __ballot(overflow);
     if (lane_id() == 0 && ov) atomicAdd(&stats->overflow, 1ull);
 }
@@ -437,8 +443,8 @@ __global__ void k_bin(WavePool B, int cur) {
     B.q_class[(size_t)cls * B.capacity + slot] = p;
 }
 
-// Untextured: held to 128 VGPRs (four waves per SIMD; two spilled registers)
-// measured 1% faster than 130 VGPRs at three waves.  Textured: 188 VGPRs at
+// Untextured: held to 128 VGPRs (four waves per SIMD; twelve spilled
+// registers) measured 1.5% faster than 136 VGPRs at three waves.  Textured: 188 VGPRs at
 // two waves; holding it to three waves spills 84 registers and gains nothing.
 template <bool TEX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4))) void k_shade(
