@@ -592,6 +592,10 @@ def main():
                              "what": "VALU issue with divergent lanes (hardware counters, see traffic source)"}
                             if kprof else None),
                 "traffic_source": (os.path.relpath(PMC_PROFILE, ROOT) if kprof else None)}
+    if ref_per_launch is not None:
+        roofline["note"] = (f"reference-layout bytes: the events the reference's binary BVH traversal of the same rays "
+                            f"would fetch; this build's four-wide BVH fetches {build_per_launch / ref_per_launch:.2f} "
+                            f"of them, so the reference-layout rate can exceed the HBM peak")
     shade = shade_roofline(st, pixels * spp, kt, prof)
     mega_roof = None
     if "megakernel" in kt:
