@@ -120,9 +120,12 @@ def shade_roofline(st, paths, kt, prof):
     and the material record (128); per miss the environment lookup (64)."""
     if "wave_shade" not in kt:
         return None
+    # (the totals below are over every path; the last factor keeps k_shade's share)
     ext, shd, hits = st["extension_rays"], st["shadow_rays"], st["extension_hits"]
     cont = ext - paths  # every extension ray after a path's first
     total = (ext * (108 + 40) + cont * 36 + shd * 68 + paths * 4 + hits * (232 + 128) + (ext - hits) * 64)
+    # k_shade shades the hits of the rays k_extend traced (the drain the rest)
+    total *= (ext - st["drain_extension_rays"]) / max(1, ext)
     launches, ms = kt["wave_shade"]
     per_launch = total / launches
     avg_s = ms / 1e3 / launches
@@ -560,9 +563,13 @@ def main():
     # per ray beside the traversal: extension: queue entry 4, ray 32, hit record
     # 24, material class 4; shadow: queue entry 4, ray 32, NEE candidate 32,
     # radiance read and write 32
+    # the drain kernel traces the last paths of a group: the persistent
+    # kernels' share of the per-ray events is the rays they traced themselves
+    ext_k = ext_rays - st["drain_extension_rays"]
+    shd_k = shd_rays - st["drain_shadow_rays"]
     build_bytes = {
-        "wave_extend": traversal_bytes(st, "", tri_test) + ext_rays * (4 + 32 + 24 + 4),
-        "wave_shadow": traversal_bytes(st, "shadow_", tri_test) + shd_rays * (4 + 32 + 32 + 32),
+        "wave_extend": (traversal_bytes(st, "", tri_test) / max(1, ext_rays) + (4 + 32 + 24 + 4)) * ext_k,
+        "wave_shadow": (traversal_bytes(st, "shadow_", tri_test) / max(1, shd_rays) + (4 + 32 + 32 + 32)) * shd_k,
     }
     dom = max((k for k in kt if k in build_bytes), key=lambda k: kt[k][1])
     launches, ms = kt[dom]
@@ -572,9 +579,9 @@ def main():
     if cpu is not None:
         c = cpu[4]
         if dom == "wave_extend":
-            ref_per_launch = reference_bytes(c) / max(1, c.extension_rays) * ext_rays / launches
+            ref_per_launch = reference_bytes(c) / max(1, c.extension_rays) * ext_k / launches
         else:
-            ref_per_launch = reference_bytes(c, shadow=True) / max(1, c.shadow_rays) * shd_rays / launches
+            ref_per_launch = reference_bytes(c, shadow=True) / max(1, c.shadow_rays) * shd_k / launches
     per_launch = ref_per_launch if ref_per_launch is not None else build_per_launch
     achieved = per_launch / avg_s / 1e9
     prof = measured_profile() if (args.queue == 0 and args.block == 64 and args.workload == "terrain") else None
@@ -588,6 +595,7 @@ def main():
                 "achieved_build_layout": round(build_per_launch / avg_s / 1e9, 1),
                 "bytes_per_launch_build_layout": int(build_per_launch),
                 "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches,
+                "rays_per_launch": int((ext_k if dom == "wave_extend" else shd_k) / launches),
                 "limiter": ({"valu_busy": kprof["valu_busy"], "lanes_active": kprof["lanes_active"],
                              "what": "VALU issue with divergent lanes (hardware counters, see traffic source)"}
                             if kprof else None),
@@ -641,6 +649,8 @@ def main():
             "shadow": {k: round(st["shadow_" + k] / max(1, shd_rays), 3)
                        for k in ("node_visits", "tri_tests", "tri_second_fetch", "instance_entries")},
             "extension_hit_fraction": round(st["extension_hits"] / max(1, ext_rays), 4),
+            "drained_fraction": {"extension": round(st["drain_extension_rays"] / max(1, ext_rays), 4),
+                                 "shadow": round(st["drain_shadow_rays"] / max(1, shd_rays), 4)},
             "layout": "this build's four-wide BVH (device counters of the counting pass)"},
     }
     if t_mega is not None:

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-/* 2: noor_stats gained extension_hits */
+/* 2: noor_stats gained extension_hits and the drain's ray counts */
 #define NOOR_ABI_VERSION 2
 
 typedef enum noor_status {
@@ -272,6 +272,9 @@ typedef struct noor_stats {
     uint64_t shadow_node_visits, shadow_tri_tests, shadow_tri_second_fetch, shadow_instance_entries;
     /* extension rays that hit a triangle or an area light (the rest escape) */
     uint64_t extension_hits;
+    /* of extension_rays and shadow_rays, those the wavefront renderer's drain
+     * kernel traced (the last paths of a group, finished in one launch) */
+    uint64_t drain_extension_rays, drain_shadow_rays;
 } noor_stats;
 
 /* ---- hit queries (test hook; noor/cuda/bvh.cuh:142-234) ------------------ */

@@ -166,7 +166,7 @@ class Stats(C.Structure):
     _fields_ = [(n, c_u64) for n in (
         "extension_rays", "shadow_rays", "node_visits", "tri_tests", "tri_second_fetch",
         "instance_entries", "shadow_node_visits", "shadow_tri_tests", "shadow_tri_second_fetch",
-        "shadow_instance_entries", "extension_hits")]
+        "shadow_instance_entries", "extension_hits", "drain_extension_rays", "drain_shadow_rays")]
 
 
 class Ray(C.Structure):

@@ -937,6 +937,8 @@ noor_status noor_get_stats(noor_ctx* ctx, noor_stats* out) {
     for (const auto& sl : d.slot) {
         out->extension_rays += sl.extension_rays;
         out->shadow_rays += sl.shadow_rays;
+        out->drain_extension_rays += sl.drain_extension_rays;
+        out->drain_shadow_rays += sl.drain_shadow_rays;
     }
     out->node_visits = d.ext[0];
     out->tri_tests = d.ext[1];

@@ -20,7 +20,7 @@ namespace noor {
 constexpr int kStatSlots = 64;
 struct DevStats {
     struct Slot {
-        unsigned long long extension_rays, shadow_rays, pad[30];
+        unsigned long long extension_rays, shadow_rays, drain_extension_rays, drain_shadow_rays, pad[28];
     } slot[kStatSlots];
     unsigned long long ext[4];   // nodes, triangle tests, second Wald fetches, instance entries
     unsigned long long shd[4];

@@ -123,6 +123,37 @@ def test_wavefront_matches_megakernel(cornell_small, graph, monkeypatch):
     assert sa["shadow_rays"] == sb["shadow_rays"]
 
 
+@pytest.mark.parametrize("capacity", [0, 1024])
+def test_wavefront_counters_equal_megakernel(cornell_small, capacity):
+    """Both renderers trace the same rays through the same traversal code, so
+    with counting on every counter agrees: rays, hits, node visits, triangle
+    tests and instance entries, the rays the wavefront's drain kernel traced
+    (the last paths of each group) included."""
+    wl = cornell_small
+    wl.options.stats = 1
+    bounces = wl.spec.bounces
+    wl.spec.bounces = 8  # paths outlive a replay of iterations, so groups end in the drain
+    try:
+        a = Renderer.from_workload(wl)
+        a.render(spp=3)
+        b = Renderer.from_workload(wl)
+        b.set_wavefront_params(queue_capacity=capacity)
+        b.render_wavefront(spp=3)
+    finally:
+        wl.options.stats = 0
+        wl.spec.bounces = bounces
+    np.testing.assert_array_equal(b.film(), a.film())
+    sa, sb = a.stats(), b.stats()
+    print({k: (sa[k], sb[k]) for k in sa})
+    for k in ("extension_rays", "shadow_rays", "extension_hits", "node_visits", "tri_tests", "instance_entries",
+              "shadow_node_visits", "shadow_tri_tests", "shadow_instance_entries"):
+        assert sa[k] == sb[k], k
+    assert sa["drain_extension_rays"] == 0 and sa["drain_shadow_rays"] == 0  # the megakernel has no drain
+    assert 0 < sb["drain_extension_rays"] < sb["extension_rays"]
+    assert sb["drain_shadow_rays"] <= sb["shadow_rays"]
+    assert 0 < sb["extension_hits"] < sb["extension_rays"]
+
+
 def test_wavefront_small_queue_chunks(cornell_small):
     """Queue capacity below the pixel count: the frame is rendered in chunks."""
     wl = cornell_small
