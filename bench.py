@@ -23,7 +23,8 @@ instance entry (per extension ray, scaled to the step's extension rays).
 compressed wide-node visit, 48 B per watertight triangle test, 48 B per
 instance entry, 80 B of queue and record traffic per ray).  `traffic` and
 `limiter` are the HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, scaled by a
-copy of known size) and the VALU activity of the same kernel, read from
+copy of known size) and the VALU rate, lane activity and wave waiting of the
+same kernel, read from
 profiles/pmc_traffic_terrain_1080p.md when that profile was measured on the
 current kernel sources (matched by a hash of noor_amd/csrc, include and the
 build flags), else null.
@@ -596,8 +597,10 @@ def main():
                 "bytes_per_launch_build_layout": int(build_per_launch),
                 "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches,
                 "rays_per_launch": int((ext_k if dom == "wave_extend" else shd_k) / launches),
-                "limiter": ({"valu_busy": kprof["valu_busy"], "lanes_active": kprof["lanes_active"],
-                             "what": "VALU issue with divergent lanes (hardware counters, see traffic source)"}
+                "limiter": ({"valu_rate": kprof["valu_rate"], "lanes_active": kprof["lanes_active"],
+                             "wave_wait_fraction": kprof["wave_wait_fraction"],
+                             "what": "latency of the traversal's dependent fetches with a third of the lanes active "
+                                     "(hardware counters, see traffic source)"}
                             if kprof else None),
                 "traffic_source": (os.path.relpath(PMC_PROFILE, ROOT) if kprof else None)}
     if ref_per_launch is not None:
