@@ -307,12 +307,19 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
         B.q_free[nxt][t] = p;
         return;
     }
-    // segment of this item: segments are sorted by item_begin
-    uint32_t lo = 0, hi = ctl->num_segments - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (B.segs[mid].item_begin <= item) lo = mid;
-        else hi = mid - 1;
+    // segment of this item: with one view the segments are its sample
+    // layers, equal in size; otherwise a search (segments are sorted by
+    // item_begin), whose dependent loads one view does without
+    uint32_t lo = 0;
+    if (ctl->num_views == 1) {
+        lo = item / (B.views[0].cam.w * B.views[0].cam.h);
+    } else {
+        uint32_t hi = ctl->num_segments - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (B.segs[mid].item_begin <= item) lo = mid;
+            else hi = mid - 1;
+        }
     }
     const Segment sg = B.segs[lo];
     const ViewDesc& V = B.views[sg.view];
@@ -715,9 +722,11 @@ __global__ void k_finish(WavePool B, float4* films, uint32_t s0, uint32_t s1, ui
     if (s0 >= end) return;
     float4* f = films + V.film_offset + px;
     float4 acc = *f;
+    // one view: the group's segments are its sample layers in order
+    const uint32_t layer = nv == 1 ? V.cam.w * V.cam.h : 0u;
     for (uint32_t s = s0; s < end; ++s) {
-        const int32_t sg = B.seg_of[(size_t)s * nv + lo];
-        const float4 c = B.sb[B.segs[sg].item_begin + px];
+        const size_t item = nv == 1 ? (size_t)(s - s0) * layer : B.segs[B.seg_of[(size_t)s * nv + lo]].item_begin;
+        const float4 c = B.sb[item + px];
         acc = film_lerp(acc, f3{c.x, c.y, c.z}, V.frame0 + s);
     }
     *f = acc;

@@ -82,6 +82,27 @@ def test_textured_batch_with_mis(mis):
         np.testing.assert_array_equal(r.batch_film(i), m.film())
 
 
+@pytest.mark.parametrize("mis", [False, True])
+def test_drain_of_textured_paths_with_mis(mis):
+    """The last paths of each group finished by the drain kernel (eight
+    bounces and a pool smaller than the frame, so paths are in flight when
+    the last items start), in the textured instantiation with and without
+    MIS: identical to the megakernel, and the drain traced some rays."""
+    tw = scenes.textured(mis=mis, bounces=8)
+    tw.options.stats = 1
+    r = Renderer.from_workload(tw)
+    r.set_wavefront_params(queue_capacity=2048)
+    r.render_wavefront(spp=3, frame=1)
+    st = r.stats()
+    m = Renderer.from_workload(tw)
+    m.render(spp=3, frame=1)
+    np.testing.assert_array_equal(r.film(), m.film())
+    sm = m.stats()
+    print({k: (st[k], sm[k]) for k in ("extension_rays", "shadow_rays", "drain_extension_rays", "drain_shadow_rays")})
+    assert st["extension_rays"] == sm["extension_rays"] and st["shadow_rays"] == sm["shadow_rays"]
+    assert st["drain_extension_rays"] > 0
+
+
 def test_autotune_picks_a_measured_setting(wl):
     cams = _cams()
     r = Renderer.from_workload(wl)
