@@ -63,12 +63,24 @@ struct Segment {
     uint32_t pad;
 };
 
+// A queue counter on a 256-byte line of its own: every block of a launch adds
+// to the counters, and returning atomics on one line complete one at a time
+// (about 11 ns each on MI355X, measured).
+struct alignas(256) Ctr {
+    uint32_t v;
+    __host__ __device__ operator uint32_t() const { return v; }
+    __host__ __device__ Ctr& operator=(uint32_t x) {
+        v = x;
+        return *this;
+    }
+};
+
 struct WaveCtl {
-    uint32_t ray_count[2];
-    uint32_t free_count[2];
-    uint32_t class_count[kClasses];
-    uint32_t shadow_count;
-    uint32_t work_extend, work_shadow;
+    Ctr ray_count[2];
+    Ctr free_count[2];
+    Ctr class_count[kClasses];
+    Ctr shadow_count;
+    Ctr work_extend, work_shadow;
     uint32_t item_next[2];  // next item to start, by iteration parity
     uint32_t new_rays[2];   // rays k_regen appended after the continuing ones
     uint32_t item_end;      // items of the current group
@@ -381,7 +393,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     uint32_t p = 0, qi = 0;
     ClosestTrav t;
     for (;;) {
-        const uint32_t idx = refill(&ctl->work_extend, count, have, exhausted);
+        const uint32_t idx = refill(&ctl->work_extend.v, count, have, exhausted);
         if (idx != 0xFFFFFFFFu) {
             p = q[idx];
             qi = idx;
@@ -439,7 +451,7 @@ __global__ void k_bin(WavePool B, int cur) {
             cnt[c][w] = tot;
             tot += n;
         }
-        const uint32_t base = tot ? atomicAdd(&ctl->class_count[c], tot) : 0u;
+        const uint32_t base = tot ? atomicAdd(&ctl->class_count[c].v, tot) : 0u;
         for (uint32_t w = 0; w < nw; ++w) cnt[c][w] += base;
     }
     __syncthreads();
@@ -532,7 +544,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
         store_state(B, p, ps);
     }
     uint32_t s1, s2, s3;
-    block_append3(&ctl->shadow_count, shadow, &ctl->ray_count[nxt], cont, &ctl->free_count[nxt], ended, s1, s2, s3);
+    block_append3(&ctl->shadow_count.v, shadow, &ctl->ray_count[nxt].v, cont, &ctl->free_count[nxt].v, ended, s1, s2,
+                  s3);
     if (shadow) {
         B.q_shadow[s1] = p;
         B.sh_o[s1] = sho;
@@ -564,7 +577,7 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WavePool B, DevStats* 
     bool second = false, occ_a = false;
     AnyTrav t;
     for (;;) {
-        const uint32_t idx = refill(&ctl->work_shadow, count, have, exhausted);
+        const uint32_t idx = refill(&ctl->work_shadow.v, count, have, exhausted);
         if (idx != 0xFFFFFFFFu) {
             p = B.q_shadow[idx];
             occ_a = false;
@@ -628,7 +641,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     extern __shared__ uint32_t smem[];
     const WaveCtl* ctl = B.ctl;
     const TravStack st = make_stack(S, smem);
-    const uint32_t nray = ctl->ray_count[0], nfree = ctl->free_count[0], n = max(nray, nfree);
+    const uint32_t nray = ctl->ray_count[0], nfree = ctl->free_count[0], n = nray > nfree ? nray : nfree;
     Counters cnt{0, 0, 0, 0}, cns{0, 0, 0, 0};
     bool overflow = false;
     uint32_t ext = 0, shd = 0, nhit = 0;
@@ -1003,7 +1016,7 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
             if (h.item_next[0] >= h.item_end && h.ray_count[0] == 0 && h.free_count[0] == 0) break;
             if (h.item_next[0] >= h.item_end && h.ray_count[0] <= kDrainMax && c.S.bounces > 0) {
                 // every item has started and few paths are left: finish them in one launch
-                const uint32_t n = std::max(h.ray_count[0], h.free_count[0]);
+                const uint32_t n = std::max<uint32_t>(h.ray_count[0], h.free_count[0]);
                 const uint32_t blocks = std::min<uint32_t>((n + 63) / 64, c.S.gstack_lanes / 64u);
                 const size_t dlds = (size_t)64 * c.S.stack_lds * sizeof(uint32_t);
                 auto drain = c.opt.stats ? (c.S.textured ? k_drain<true, true> : k_drain<true, false>)
