@@ -424,48 +424,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     if (lane_id() == 0 && ov) atomicAdd(&stats->overflow, 1ull);
 }
 
-// Material-class binning of the extension queue, in queue order: one LDS
-// histogram and one atomic per class per block (blocks of up to 16 waves).
+// Material-class binning of the extension queue, in queue order: a block of
+// up to 16 waves takes kBinPer consecutive stretches of blockDim.x entries,
+// counts them per class in LDS and claims its slots with one atomic per class
+// (returning atomics on one counter complete about one per 11 ns, and with one
+// stretch per block k_bin ran at that rate).
+constexpr int kBinPer = 4;
 __global__ void k_bin(WavePool B, int cur) {
     WaveCtl* ctl = B.ctl;
     const uint32_t count = ctl->ray_count[cur] + ctl->new_rays[cur];
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x * blockDim.x >= count) return;  // whole block idle (uniform per block)
-    __shared__ uint32_t cnt[kClasses][16];
+    const uint32_t first = blockIdx.x * blockDim.x * kBinPer;
+    if (first >= count) return;  // whole block idle (uniform per block)
+    __shared__ uint32_t cnt[kClasses][16 * kBinPer];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const bool valid = t < count;
-    const uint32_t p = valid ? B.q_ray[cur][t] : 0u;
-    const int cls = valid ? (int)B.cls[p] : -1;
-    unsigned long long m[kClasses];
+    uint32_t p[kBinPer];
+    int cls[kBinPer];
 #pragma unroll
-    for (int c = 0; c < kClasses; ++c) {
-        m[c] = __ballot(cls == c);
-        if (lane == 0) cnt[c][wave] = (uint32_t)__popcll(m[c]);
+    for (int j = 0; j < kBinPer; ++j) {
+        const uint32_t t = first + (uint32_t)j * blockDim.x + threadIdx.x;
+        const bool valid = t < count;
+        p[j] = valid ? B.q_ray[cur][t] : 0u;
+        cls[j] = valid ? (int)B.cls[p[j]] : -1;
     }
+#pragma unroll
+    for (int j = 0; j < kBinPer; ++j)
+#pragma unroll
+        for (int c = 0; c < kClasses; ++c) {
+            const unsigned long long m = __ballot(cls[j] == c);
+            if (lane == 0) cnt[c][j * nw + wave] = (uint32_t)__popcll(m);
+        }
     __syncthreads();
     if (threadIdx.x < kClasses) {
+        // stretches in queue order, waves in order within a stretch
         const int c = threadIdx.x;
+        const uint32_t n_parts = nw * kBinPer;
         uint32_t tot = 0;
-        for (uint32_t w = 0; w < nw; ++w) {
+        for (uint32_t w = 0; w < n_parts; ++w) {
             const uint32_t n = cnt[c][w];
             cnt[c][w] = tot;
             tot += n;
         }
         const uint32_t base = tot ? atomicAdd(&ctl->class_count[c].v, tot) : 0u;
-        for (uint32_t w = 0; w < nw; ++w) cnt[c][w] += base;
+        for (uint32_t w = 0; w < n_parts; ++w) cnt[c][w] += base;
     }
     __syncthreads();
-    if (cls < 0) return;
-    unsigned long long mine = 0;
-    uint32_t base = 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int c = 0; c < kClasses; ++c)
-        if (c == cls) {
-            mine = m[c];
-            base = cnt[c][wave];
+    for (int j = 0; j < kBinPer; ++j)
+#pragma unroll
+        for (int c = 0; c < kClasses; ++c) {
+            const unsigned long long m = __ballot(cls[j] == c);
+            if (cls[j] == c)
+                B.q_class[(size_t)c * B.capacity + cnt[c][j * nw + wave] + (uint32_t)__popcll(m & below)] = p[j];
         }
-    const uint32_t slot = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
-    B.q_class[(size_t)cls * B.capacity + slot] = p;
 }
 
 // Untextured: held to 128 VGPRs (four waves per SIMD; twelve spilled
@@ -824,9 +835,9 @@ static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
     const uint32_t cap = B.P.capacity;
     const dim3 g1((cap + 255) / 256), b1(256);
     // k_bin takes one atomic per class per block: with returning atomics on
-    // one address completing about one per 11 ns (measured), 256-lane blocks
-    // left it bound by them; 1024-lane blocks cut its time to a third
-    const dim3 gb((cap + 1023) / 1024), bb(1024);
+    // one address completing about one per 11 ns (measured), blocks of 256
+    // entries left it bound by them; 4096 entries per block (1024 lanes)
+    const dim3 gb((cap + 1024 * kBinPer - 1) / (1024 * kBinPer)), bb(1024);
     const int blk = B.P.block;
     const size_t lds = (size_t)blk * B.P.stack_lds * sizeof(uint32_t);
     const dim3 bp(blk);
