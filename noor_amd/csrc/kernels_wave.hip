@@ -103,8 +103,6 @@ struct WavePool {
     uint32_t max_views = 0, max_segments = 0;
     // path state (one slot per path in flight)
     uint32_t* item = nullptr;
-    uint32_t* pix = nullptr;
-    uint32_t* view = nullptr;
     float4* L = nullptr;       // L.xyz
     float4* beta = nullptr;    // beta.xyz, w = bounce | specular << 16
     uint64_t* rng = nullptr;
@@ -347,8 +345,6 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
     const Ray r = gen_ray<TEX>(V.cam, x, y, ps.rng, S.rng_kind, &rdf);
     if (TEX) store_diff(B, p, rdf);
     B.item[p] = item;
-    B.pix[p] = gid;
-    B.view[p] = sg.view;
     B.rng_inc[p] = ps.rng.inc;
     store_state(B, p, ps);
     B.q_ray[cur][c0 + t] = p;
@@ -771,8 +767,6 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
 #define WA(ptr, n) \
     if (e == hipSuccess) e = alloc((void**)&(ptr), sizeof(*(ptr)) * (size_t)(n));
     WA(P.item, cap);
-    WA(P.pix, cap);
-    WA(P.view, cap);
     WA(P.L, cap);
     WA(P.beta, cap);
     WA(P.rng, cap);
