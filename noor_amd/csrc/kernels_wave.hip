@@ -115,7 +115,9 @@ struct WavePool {
     float4* ddy = nullptr;
     bool tex = false;
     float4* hit = nullptr;     // t, u, v, prim
-    uint2* hit2 = nullptr;     // inst, kind | extension-queue position << 2
+    // inst, kind | fresh << 2 | extension-queue position << 3 (fresh: the
+    // path's first ray, whose L and beta k_regen leaves unwritten)
+    uint2* hit2 = nullptr;
     uint32_t* cls = nullptr;   // material class of the hit (k_extend -> k_bin)
     float4* nee_a = nullptr;   // candidate.xyz
     float4* nee_beta = nullptr;
@@ -346,9 +348,11 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
     if (TEX) store_diff(B, p, rdf);
     B.item[p] = item;
     B.rng_inc[p] = ps.rng.inc;
-    store_state(B, p, ps);
+    // L = 0, beta = 1, bounce 0: k_shade knows a path's first ray by the w of
+    // its queued origin (carried in the hit record), so only the RNG is stored
+    B.rng[p] = ps.rng.state;
     B.q_ray[cur][c0 + t] = p;
-    B.qo[cur][c0 + t] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+    B.qo[cur][c0 + t] = make_float4(r.o.x, r.o.y, r.o.z, 1.0f);
     B.qd[cur][c0 + t] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
 }
 
@@ -392,8 +396,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const uint32_t idx = refill(&ctl->work_extend.v, count, have, exhausted);
         if (idx != 0xFFFFFFFFu) {
             p = q[idx];
-            qi = idx;
             const float4 o = B.qo[cur][idx], d = B.qd[cur][idx];
+            qi = (idx << 3) | (o.w != 0.0f ? 4u : 0u);
             closest_begin(S, mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, K_INF), t);
             have = true;
             ++ext;
@@ -401,7 +405,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         if (__ballot(have) == 0ull) break;
         if (have && closest_steps<STATS, ALPHA>(S, t, st, cnt, overflow, B.steps)) {
             B.hit[p] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.prim));
-            B.hit2[p] = make_uint2(t.h.inst, (t.hit ? t.h.kind : 0u) | (qi << 2));
+            B.hit2[p] = make_uint2(t.h.inst, (t.hit ? t.h.kind : 0u) | qi);
             B.cls[p] = (uint32_t)material_class(S, t.hit, t.h);
             if (STATS && t.hit) ++nhit;
             have = false;
@@ -503,11 +507,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
     float4 sho, shd, nro, nrd;  // the shadow ray and the next ray, written at their queue positions
     if (cls >= 0) {
         p = B.q_class[(size_t)cls * B.capacity + local];
-        PathState ps;
-        load_state(B, p, ps);
         const float4 hh = B.hit[p];
         const uint2 h2 = B.hit2[p];
-        const float4 o = B.qo[cur][h2.y >> 2], d = B.qd[cur][h2.y >> 2];
+        PathState ps;
+        if (h2.y & 4u) {  // the path's first ray: k_regen stored only its RNG
+            ps.L = f3{0.0f, 0.0f, 0.0f};
+            ps.beta = f3{1.0f, 1.0f, 1.0f};
+            ps.bounce = 0;
+            ps.specular = 0;
+            ps.rng.state = B.rng[p];
+            ps.rng.inc = B.rng_inc[p];
+        } else {
+            load_state(B, p, ps);
+        }
+        const float4 o = B.qo[cur][h2.y >> 3], d = B.qd[cur][h2.y >> 3];
         Ray ray = mkray(f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, K_INF);
         Hit h;
         h.t = hh.x;
