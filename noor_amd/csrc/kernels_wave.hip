@@ -561,7 +561,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
             nrd = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
             if (TEX) store_diff(B, p, rdf);
         }
-        store_state(B, p, ps);
+        if (ended && !shadow) {
+            // done, with no occlusion test pending: the sample goes to the
+            // sample buffer now and the slot is retired (k_regen then finds
+            // kNoItem), instead of storing the state for k_regen to read back
+            const uint32_t it = B.item[p];
+            const f3 c = final_radiance(S, ps.L);
+            B.sb[it] = make_float4(c.x, c.y, c.z, 0.0f);
+            B.item[p] = kNoItem;
+        } else {
+            store_state(B, p, ps);
+        }
     }
     uint32_t s1, s2, s3;
     block_append3(&ctl->shadow_count.v, shadow, &ctl->ray_count[nxt].v, cont, &ctl->free_count[nxt].v, ended, s1, s2,
