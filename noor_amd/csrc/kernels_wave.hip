@@ -114,10 +114,11 @@ struct WavePool {
     float4* ddx = nullptr;
     float4* ddy = nullptr;
     bool tex = false;
-    float4* hit = nullptr;     // t, u, v, prim
-    // inst, kind | fresh << 2 | extension-queue position << 3 (fresh: the
-    // path's first ray, whose L and beta k_regen leaves unwritten)
-    uint2* hit2 = nullptr;
+    // hit record, two uint4 per slot (32 bytes, one memory line): t, u, v,
+    // prim; then instance, kind | fresh << 2 | extension-queue position << 3
+    // (fresh: the path's first ray, whose L and beta k_regen leaves
+    // unwritten)
+    uint4* hit = nullptr;
     uint32_t* cls = nullptr;   // material class of the hit (k_extend -> k_bin)
     float4* nee_a = nullptr;   // candidate.xyz
     float4* nee_beta = nullptr;
@@ -404,8 +405,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         }
         if (__ballot(have) == 0ull) break;
         if (have && closest_steps<STATS, ALPHA>(S, t, st, cnt, overflow, B.steps)) {
-            B.hit[p] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.prim));
-            B.hit2[p] = make_uint2(t.h.inst, (t.hit ? t.h.kind : 0u) | qi);
+            B.hit[2 * (size_t)p] = make_uint4(__float_as_uint(t.h.t), __float_as_uint(t.h.u), __float_as_uint(t.h.v),
+                                              t.h.prim);
+            B.hit[2 * (size_t)p + 1] = make_uint4(t.h.inst, (t.hit ? t.h.kind : 0u) | qi, 0u, 0u);
             B.cls[p] = (uint32_t)material_class(S, t.hit, t.h);
             if (STATS && t.hit) ++nhit;
             have = false;
@@ -507,8 +509,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
     float4 sho, shd, nro, nrd;  // the shadow ray and the next ray, written at their queue positions
     if (cls >= 0) {
         p = B.q_class[(size_t)cls * B.capacity + local];
-        const float4 hh = B.hit[p];
-        const uint2 h2 = B.hit2[p];
+        const uint4 h0 = B.hit[2 * (size_t)p], h2 = B.hit[2 * (size_t)p + 1];
+        const float4 hh = make_float4(__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z),
+                                      __uint_as_float(h0.w));
         PathState ps;
         if (h2.y & 4u) {  // the path's first ray: k_regen stored only its RNG
             ps.L = f3{0.0f, 0.0f, 0.0f};
@@ -801,8 +804,7 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
         WA(P.ddy, cap);
     }
     P.tex = tex;
-    WA(P.hit, cap);
-    WA(P.hit2, cap);
+    WA(P.hit, 2 * (size_t)cap);
     WA(P.cls, cap);
     WA(P.nee_a, cap);
     WA(P.nee_beta, cap);
