@@ -113,18 +113,20 @@ def reference_bytes(cnt, shadow=False):
 def shade_roofline(st, paths, kt, prof):
     """k_shade's algorithmic bytes per launch and its achieved rate.  Per
     shaded path: path state read (class-queue entry 4, L 16, beta and bounce
-    16, RNG 16, hit record 24, ray 32) and written (L, beta, RNG: 40); per
-    continuing ray the next ray and its queue entry (36); per shadow task the
-    NEE candidate, shadow ray and queue entry (68); per ended path its
-    free-queue entry (4); per hit the shading fetch (attributes 16, three
-    vertices, normals and uvs 120, object-to-world and normal rows 96: 232)
-    and the material record (128); per miss the environment lookup (64)."""
+    16, RNG 16, hit record 24, ray 32; a path's first ray reads no L and
+    beta) and written (L, beta, RNG: 40); per continuing ray the next ray and
+    its queue entry (36); per shadow task the NEE candidate, shadow ray and
+    queue entry (68); per ended path its free-queue entry (4); per hit the
+    shading fetch (attributes 16, three vertices, normals and uvs 120,
+    object-to-world and normal rows 96: 232) and the material record (128);
+    per miss the environment lookup (64)."""
     if "wave_shade" not in kt:
         return None
     # (the totals below are over every path; the last factor keeps k_shade's share)
     ext, shd, hits = st["extension_rays"], st["shadow_rays"], st["extension_hits"]
     cont = ext - paths  # every extension ray after a path's first
-    total = (ext * (108 + 40) + cont * 36 + shd * 68 + paths * 4 + hits * (232 + 128) + (ext - hits) * 64)
+    total = (ext * (108 + 40) - paths * 32 + cont * 36 + shd * 68 + paths * 4 + hits * (232 + 128) +
+             (ext - hits) * 64)
     # k_shade shades the hits of the rays k_extend traced (the drain the rest)
     total *= (ext - st["drain_extension_rays"]) / max(1, ext)
     launches, ms = kt["wave_shade"]
