@@ -19,7 +19,7 @@ bytes count the events of the reference's binary-BVH traversal of the same
 rays, from the oracle's counters over the CPU sample: 32 B per node visit,
 32 B per triangle test plus 16 B when the second Wald record is read, 48 B per
 instance entry (per extension ray, scaled to the step's extension rays).
-`achieved_build_layout` counts this build's own fetches instead (56 B per
+`achieved_build_layout` counts the renderer's own fetches instead (56 B per
 compressed wide-node visit, 48 B per watertight triangle test, 48 B per
 instance entry, 80 B of queue and record traffic per ray).  `traffic` and
 `limiter` are the HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, scaled by a
@@ -93,7 +93,7 @@ def measured_profile():
 
 
 def traversal_bytes(st, prefix, tri_test):
-    """This build's own fetches (compressed wide nodes)."""
+    """The renderer's own fetches (compressed wide nodes)."""
     nodes = st[prefix + "node_visits"]
     tris = st[prefix + "tri_tests"]
     inst = st[prefix + "instance_entries"]
@@ -466,7 +466,7 @@ def main():
         print(json.dumps({"textured": bench_textured()}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the renderer has no multi-GPU path "
               "(pixels shard trivially; see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
@@ -594,7 +594,7 @@ def main():
                 "traffic": kprof["hbm_bytes_per_dispatch"] if kprof else None,
                 "bytes_per_launch": int(per_launch),
                 "bytes_layout": "reference (binary BVH, Wald records; oracle-counted events)"
-                if ref_per_launch is not None else "this build's wide BVH (no CPU sample to count the reference's)",
+                if ref_per_launch is not None else "the renderer's wide BVH (no CPU sample to count the reference's)",
                 "achieved_build_layout": round(build_per_launch / avg_s / 1e9, 1),
                 "bytes_per_launch_build_layout": int(build_per_launch),
                 "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches,
@@ -607,7 +607,7 @@ def main():
                 "traffic_source": (os.path.relpath(PMC_PROFILE, ROOT) if kprof else None)}
     if ref_per_launch is not None:
         roofline["note"] = (f"reference-layout bytes: the events the reference's binary BVH traversal of the same rays "
-                            f"would fetch; this build's four-wide BVH fetches {build_per_launch / ref_per_launch:.2f} "
+                            f"would fetch; the renderer's four-wide BVH fetches {build_per_launch / ref_per_launch:.2f} "
                             f"of them, so the reference-layout rate can exceed the HBM peak")
     shade = shade_roofline(st, pixels * spp, kt, prof)
     mega_roof = None
@@ -656,7 +656,7 @@ def main():
             "extension_hit_fraction": round(st["extension_hits"] / max(1, ext_rays), 4),
             "drained_fraction": {"extension": round(st["drain_extension_rays"] / max(1, ext_rays), 4),
                                  "shadow": round(st["drain_shadow_rays"] / max(1, shd_rays), 4)},
-            "layout": "this build's four-wide BVH (device counters of the counting pass)"},
+            "layout": "the renderer's four-wide BVH (device counters of the counting pass)"},
     }
     if t_mega is not None:
         mega_mrays = rays_per_step * args.steps / t_mega / 1e6

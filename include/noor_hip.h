@@ -8,7 +8,7 @@
  * argument is a POD struct or a pointer + count, and every call returns a
  * status code.  The PODs keep the field order and meaning of the reference
  * wire format so that a host which fills CudaPayload / CudaSpec / CudaCamera /
- * CudaHosekSky can fill these instead (INTEGRATION.md shows the copy).
+ * CudaHosekSky can fill these instead, field by field.
  *
  * Mapping of the reference entry points:
  *   load_cuda_data      (noor/cuda/path_tracer_data.cuh:53-62)  -> noor_create

@@ -1,7 +1,8 @@
 """ctypes binding of oracle/liboracle.so (TEST INFRASTRUCTURE).
 
 Used only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
-as the checker or the CPU baseline.  Never imported by noor_amd/.
+as the reference results of parity checks or as the CPU baseline.  Never
+imported by noor_amd/.
 """
 import ctypes as C
 import os

@@ -2,8 +2,8 @@
  * oracle.h -- CPU restatement of the reference render path (TEST INFRASTRUCTURE).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load the
- * library built from this directory, and only as the checker or the CPU
- * baseline.  The product (noor_amd/) never links or calls it.
+ * library built from this directory, and only as the reference results of
+ * parity checks or as the CPU baseline.  The product (noor_amd/) never links or calls it.
  *
  * The restatement follows the reference files named in each function's
  * comment.  It consumes the same wire format as the product's C ABI
