@@ -103,10 +103,11 @@ struct WavePool {
     uint32_t max_views = 0, max_segments = 0;
     // path state (one slot per path in flight)
     uint32_t* item = nullptr;
-    float4* L = nullptr;       // L.xyz
-    float4* beta = nullptr;    // beta.xyz, w = bounce | specular << 16
+    // two records per slot, each pair in one 32-byte sector: L.xyz, then
+    // beta.xyz with w = bounce | specular << 16; the RNG state, then its
+    // increment
+    float4* Lb = nullptr;
     uint64_t* rng = nullptr;
-    uint64_t* rng_inc = nullptr;
     // ray differentials of textured scenes: auxiliary origins (w of dox = has)
     // and directions
     float4* dox = nullptr;
@@ -221,19 +222,19 @@ __device__ __forceinline__ int material_class(const DevScene& S, bool hit, const
 }
 
 __device__ __forceinline__ void load_state(const WavePool& B, uint32_t p, PathState& ps) {
-    const float4 L = B.L[p], bt = B.beta[p];
+    const float4 L = B.Lb[2 * (size_t)p], bt = B.Lb[2 * (size_t)p + 1];
     ps.L = f3{L.x, L.y, L.z};
     ps.beta = f3{bt.x, bt.y, bt.z};
     const uint32_t packed = __float_as_uint(bt.w);
     ps.bounce = packed & 0xFFFFu;
     ps.specular = packed >> 16;
-    ps.rng.state = B.rng[p];
-    ps.rng.inc = B.rng_inc[p];
+    ps.rng.state = B.rng[2 * (size_t)p];
+    ps.rng.inc = B.rng[2 * (size_t)p + 1];
 }
 __device__ __forceinline__ void store_state(const WavePool& B, uint32_t p, const PathState& ps) {
-    B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
-    B.beta[p] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, __uint_as_float(ps.bounce | (ps.specular << 16)));
-    B.rng[p] = ps.rng.state;
+    B.Lb[2 * (size_t)p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+    B.Lb[2 * (size_t)p + 1] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, __uint_as_float(ps.bounce | (ps.specular << 16)));
+    B.rng[2 * (size_t)p] = ps.rng.state;
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -305,7 +306,7 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
     const uint32_t p = B.q_free[cur][t];
     const uint32_t it = B.item[p];
     if (it != kNoItem) {
-        const float4 L = B.L[p];
+        const float4 L = B.Lb[2 * (size_t)p];
         const f3 c = final_radiance(S, f3{L.x, L.y, L.z});
         B.sb[it] = make_float4(c.x, c.y, c.z, 0.0f);
         B.item[p] = kNoItem;
@@ -348,10 +349,10 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
     const Ray r = gen_ray<TEX>(V.cam, x, y, ps.rng, S.rng_kind, &rdf);
     if (TEX) store_diff(B, p, rdf);
     B.item[p] = item;
-    B.rng_inc[p] = ps.rng.inc;
+    B.rng[2 * (size_t)p + 1] = ps.rng.inc;
     // L = 0, beta = 1, bounce 0: k_shade knows a path's first ray by the w of
     // its queued origin (carried in the hit record), so only the RNG is stored
-    B.rng[p] = ps.rng.state;
+    B.rng[2 * (size_t)p] = ps.rng.state;
     B.q_ray[cur][c0 + t] = p;
     B.qo[cur][c0 + t] = make_float4(r.o.x, r.o.y, r.o.z, 1.0f);
     B.qd[cur][c0 + t] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
@@ -518,8 +519,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
             ps.beta = f3{1.0f, 1.0f, 1.0f};
             ps.bounce = 0;
             ps.specular = 0;
-            ps.rng.state = B.rng[p];
-            ps.rng.inc = B.rng_inc[p];
+            ps.rng.state = B.rng[2 * (size_t)p];
+            ps.rng.inc = B.rng[2 * (size_t)p + 1];
         } else {
             load_state(B, p, ps);
         }
@@ -645,10 +646,10 @@ __global__ void k_shadow(const DevScene* __restrict__ Sp, WavePool B, DevStats* 
             const bool oa = nee.na ? (second && nb ? occ_a : t.occluded) : false;
             const bool ob = nb ? t.occluded : false;
             PathState ps;
-            const float4 L = B.L[p];
+            const float4 L = B.Lb[2 * (size_t)p];
             ps.L = f3{L.x, L.y, L.z};
             resolve_nee(ps, nee, oa, ob);
-            B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+            B.Lb[2 * (size_t)p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
             have = false;
         }
     }
@@ -683,7 +684,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             const uint32_t p = B.q_free[0][t];
             const uint32_t it = B.item[p];
             if (it != kNoItem) {
-                const float4 L = B.L[p];
+                const float4 L = B.Lb[2 * (size_t)p];
                 const f3 c = final_radiance(S, f3{L.x, L.y, L.z});
                 B.sb[it] = make_float4(c.x, c.y, c.z, 0.0f);
                 B.item[p] = kNoItem;
@@ -793,10 +794,8 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
 #define WA(ptr, n) \
     if (e == hipSuccess) e = alloc((void**)&(ptr), sizeof(*(ptr)) * (size_t)(n));
     WA(P.item, cap);
-    WA(P.L, cap);
-    WA(P.beta, cap);
-    WA(P.rng, cap);
-    WA(P.rng_inc, cap);
+    WA(P.Lb, 2 * (size_t)cap);
+    WA(P.rng, 2 * (size_t)cap);
     if (tex) {
         WA(P.dox, cap);
         WA(P.doy, cap);
