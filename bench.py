@@ -21,7 +21,7 @@ rays, from the oracle's counters over the CPU sample: 32 B per node visit,
 instance entry (per extension ray, scaled to the step's extension rays).
 `achieved_build_layout` counts the renderer's own fetches instead (56 B per
 compressed wide-node visit, 48 B per watertight triangle test, 48 B per
-instance entry, 80 B of queue and record traffic per ray).  `traffic` and
+instance entry, 72 B of queue and record traffic per ray).  `traffic` and
 `limiter` are the HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, scaled by a
 copy of known size) and the VALU rate, lane activity and wave waiting of the
 same kernel, read from
@@ -113,7 +113,7 @@ def reference_bytes(cnt, shadow=False):
 def shade_roofline(st, paths, kt, prof):
     """k_shade's algorithmic bytes per launch and its achieved rate.  Per
     shaded path: path state read (class-queue entry 4, L 16, beta and bounce
-    16, RNG 16, hit record 24, ray 32; a path's first ray reads no L and
+    16, RNG 16, hit record 32, ray 32; a path's first ray reads no L and
     beta) and written (L, beta, RNG: 40); per continuing ray the next ray and
     its queue entry (36); per shadow task the NEE candidate, shadow ray and
     queue entry (68); per ended path its free-queue entry (4); per hit the
@@ -125,7 +125,7 @@ def shade_roofline(st, paths, kt, prof):
     # (the totals below are over every path; the last factor keeps k_shade's share)
     ext, shd, hits = st["extension_rays"], st["shadow_rays"], st["extension_hits"]
     cont = ext - paths  # every extension ray after a path's first
-    total = (ext * (108 + 40) - paths * 32 + cont * 36 + shd * 68 + paths * 4 + hits * (232 + 128) +
+    total = (ext * (116 + 40) - paths * 32 + cont * 36 + shd * 68 + paths * 4 + hits * (232 + 128) +
              (ext - hits) * 64)
     # k_shade shades the hits of the rays k_extend traced (the drain the rest)
     total *= (ext - st["drain_extension_rays"]) / max(1, ext)
@@ -564,14 +564,14 @@ def main():
         cb, cpu_film, rows, cpu_spp, cpu_cnt = cpu_baseline(wl, args.cpu_seconds)
         cpu = (cb, cpu_film, rows, cpu_spp, cpu_cnt)
     # per ray beside the traversal: extension: queue entry 4, ray 32, hit record
-    # 24, material class 4; shadow: queue entry 4, ray 32, NEE candidate 32,
+    # 32, material class 4; shadow: queue entry 4, ray 32, NEE candidate 32,
     # radiance read and write 32
     # the drain kernel traces the last paths of a group: the persistent
     # kernels' share of the per-ray events is the rays they traced themselves
     ext_k = ext_rays - st["drain_extension_rays"]
     shd_k = shd_rays - st["drain_shadow_rays"]
     build_bytes = {
-        "wave_extend": (traversal_bytes(st, "", tri_test) / max(1, ext_rays) + (4 + 32 + 24 + 4)) * ext_k,
+        "wave_extend": (traversal_bytes(st, "", tri_test) / max(1, ext_rays) + (4 + 32 + 32 + 4)) * ext_k,
         "wave_shadow": (traversal_bytes(st, "shadow_", tri_test) / max(1, shd_rays) + (4 + 32 + 32 + 32)) * shd_k,
     }
     dom = max((k for k in kt if k in build_bytes), key=lambda k: kt[k][1])
