@@ -345,9 +345,9 @@ noor_status noor_set_timing(noor_ctx* ctx, int enable);
 noor_status noor_get_kernel_times(noor_ctx* ctx, noor_kernel_time* out, uint32_t max, uint32_t* count);
 noor_status noor_reset_kernel_times(noor_ctx* ctx);
 /* Wavefront tunables: queue capacity in paths (0 = automatic: the items of
- * the render, at most 2^25, about 8 GB of path state, halved down to 2^22
- * when the device cannot hold it) and lanes per block of the traversal
- * kernels (64, 128 or 256). */
+ * the render, at most 2^25, about 9 GB of path state, halved down to 2^22
+ * when the device cannot hold it; otherwise 64 to 2^28) and lanes per block
+ * of the traversal kernels (64, 128 or 256). */
 noor_status noor_set_wavefront_params(noor_ctx* ctx, uint32_t queue_capacity, int32_t block);
 
 /* Traversal stack entries per lane kept in LDS (the rest of the exact

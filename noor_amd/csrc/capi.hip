@@ -655,7 +655,7 @@ noor_status noor_reset_kernel_times(noor_ctx* ctx) {
 noor_status noor_set_wavefront_params(noor_ctx* ctx, uint32_t queue_capacity, int32_t block) {
     if (!ctx) return NOOR_ERR_INVALID_ARG;
     if (block != 64 && block != 128 && block != 256) return NOOR_ERR_INVALID_ARG;
-    if (queue_capacity != 0 && queue_capacity < 64) return NOOR_ERR_INVALID_ARG;
+    if (queue_capacity != 0 && (queue_capacity < 64 || queue_capacity > (1u << 28))) return NOOR_ERR_INVALID_ARG;
     ctx->c.wave_capacity = queue_capacity;
     ctx->c.wave_block = block;
     return NOOR_OK;

@@ -950,6 +950,8 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
     // rays, so fewer launches, fuller persistent waves and shorter tails.  If
     // the device cannot hold it, the pool halves down to 2^22 slots.
     const bool automatic = capacity == 0;
+    // the hit record holds a queue position in 29 bits
+    if (capacity > (1u << 28)) return hipErrorInvalidValue;
     if (automatic) capacity = (uint32_t)std::min<uint64_t>(sb_need, kMaxAutoPool);
     capacity = std::max<uint32_t>(256u, (capacity + 255) & ~255u);
     const uint32_t requested = capacity;

@@ -128,3 +128,7 @@ def test_batch_rejects_bad_views(wl):
         r.render_batch([(c, 0, 1, 0)])  # frame must be >= 1
     with pytest.raises(abi.NoorError):
         r.render_batch([(c, 1, 0, 0)])  # spp must be >= 1
+    with pytest.raises(abi.NoorError):
+        r.set_wavefront_params(queue_capacity=(1 << 28) + 1)  # above the hit record's queue positions
+    with pytest.raises(abi.NoorError):
+        r.set_wavefront_params(queue_capacity=32)  # below one wave
