@@ -1,8 +1,9 @@
 // Wavefront renderer: streaming path regeneration, material-sorted shading
 // queues, persistent traversal kernels and hipGraph-captured iterations.
 //
-// A render is a list of sample items (sample, view, pixel), sample-major.  A
-// fixed pool of path slots is kept full; each iteration runs
+// A render is a list of sample items (sample, view, pixel), sample-major (one
+// view: pixel-major within each group of sample layers).  A fixed pool of
+// path slots is kept full; each iteration runs
 //   k_regen   retires the paths that ended in the previous iteration (their
 //             radiance goes to the item's entry of the sample buffer) and
 //             starts the next items in the freed slots           -> ray queue
@@ -55,7 +56,9 @@ struct ViewDesc {
     uint32_t pad[3];
 };
 
-// Items [item_begin, item_begin + w*h) of a group are sample `sample` of view `view`.
+// Items [item_begin, item_begin + w*h) of a group are sample `sample` of view
+// `view` (with one view the items of a group interleave its layers instead:
+// k_regen and k_finish).
 struct Segment {
     uint32_t item_begin;
     uint32_t view;
@@ -321,12 +324,16 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
         B.q_free[nxt][t] = p;
         return;
     }
-    // segment of this item: with one view the segments are its sample
-    // layers, equal in size; otherwise a search (segments are sorted by
-    // item_begin), whose dependent loads one view does without
-    uint32_t lo = 0;
+    // segment (sample layer) and pixel of this item.  One view: item i is
+    // pixel i / G of the group's layer i % G (G layers), so the consecutive
+    // items of a wave's refill are the samples of one pixel, whose first rays
+    // walk the same nodes.  Several views: items are sample-major, then view,
+    // then pixel, and a search finds the segment (sorted by item_begin).
+    uint32_t lo = 0, gid;
     if (ctl->num_views == 1) {
-        lo = item / (B.views[0].cam.w * B.views[0].cam.h);
+        const uint32_t G = ctl->num_segments;
+        lo = item % G;
+        gid = item / G;
     } else {
         uint32_t hi = ctl->num_segments - 1;
         while (lo < hi) {
@@ -334,10 +341,10 @@ __global__ void k_regen(const DevScene* __restrict__ Sp, WavePool B, int cur) {
             if (B.segs[mid].item_begin <= item) lo = mid;
             else hi = mid - 1;
         }
+        gid = item - B.segs[lo].item_begin;
     }
     const Segment sg = B.segs[lo];
     const ViewDesc& V = B.views[sg.view];
-    const uint32_t gid = item - sg.item_begin;
     const uint32_t x = gid % V.cam.w, y = gid / V.cam.w;
     PathState ps;
     ps.L = f3{0.0f, 0.0f, 0.0f};
@@ -751,7 +758,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     if (lane_id() == 0 && ov) atomicAdd(&stats->overflow, 1ull);
 }
 
-// Folds samples [s0, s1) of every pixel into its film, in sample order.
+// Folds samples [s0, s1) of every pixel into its film, in sample order
+// (several views: the items of sample s of view v start at the item_begin of
+// its segment).
 __global__ void k_finish(WavePool B, float4* films, uint32_t s0, uint32_t s1, uint32_t pixels) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= pixels) return;
@@ -769,14 +778,49 @@ __global__ void k_finish(WavePool B, float4* films, uint32_t s0, uint32_t s1, ui
     if (s0 >= end) return;
     float4* f = films + V.film_offset + px;
     float4 acc = *f;
-    // one view: the group's segments are its sample layers in order
-    const uint32_t layer = nv == 1 ? V.cam.w * V.cam.h : 0u;
     for (uint32_t s = s0; s < end; ++s) {
-        const size_t item = nv == 1 ? (size_t)(s - s0) * layer : B.segs[B.seg_of[(size_t)s * nv + lo]].item_begin;
-        const float4 c = B.sb[item + px];
+        const float4 c = B.sb[(size_t)B.segs[B.seg_of[(size_t)s * nv + lo]].item_begin + px];
         acc = film_lerp(acc, f3{c.x, c.y, c.z}, V.frame0 + s);
     }
     *f = acc;
+}
+
+// The same for one view, whose group holds every layer s0..s1-1 and the
+// samples of pixel px are the consecutive items px * G + (s - s0), G = s1 - s0.
+// A block of kFinishBlock pixels reads kFinishLayers samples of each of its
+// pixels at a time into LDS (consecutive lanes read consecutive items, so a
+// wave reads whole lines) and each lane then folds its own pixel's from
+// there; a lane reading its own pixel's samples straight from memory strides
+// by G items and ran 5x slower.
+constexpr uint32_t kFinishBlock = 256;
+constexpr uint32_t kFinishLayers = 8;
+__global__ __launch_bounds__(kFinishBlock) void k_finish_one(WavePool B, float4* films, uint32_t s0, uint32_t s1,
+                                                             uint32_t pixels) {
+    __shared__ float4 tile[kFinishBlock * (kFinishLayers + 1)];  // a pixel's row padded by one entry
+    const uint32_t G = s1 - s0;
+    const uint32_t px0 = blockIdx.x * kFinishBlock;
+    const uint32_t npx = min(kFinishBlock, pixels - px0);
+    const uint32_t t = threadIdx.x;
+    const ViewDesc& V = B.views[0];
+    float4* f = films + V.film_offset + px0 + t;
+    float4 acc = t < npx ? *f : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4* src = B.sb + (size_t)px0 * G;
+    for (uint32_t c = 0; c < G; c += kFinishLayers) {
+        const uint32_t kn = min(kFinishLayers, G - c);
+        for (uint32_t e = t; e < npx * kn; e += kFinishBlock) {
+            const uint32_t q = kn == kFinishLayers ? e / kFinishLayers : e / kn;
+            const uint32_t k = e - q * kn;
+            tile[q * (kFinishLayers + 1) + k] = src[(size_t)q * G + c + k];
+        }
+        __syncthreads();
+        if (t < npx)
+            for (uint32_t k = 0; k < kn; ++k) {
+                const float4 v = tile[t * (kFinishLayers + 1) + k];
+                acc = film_lerp(acc, f3{v.x, v.y, v.z}, V.frame0 + s0 + c + k);
+            }
+        __syncthreads();
+    }
+    if (t < npx) *f = acc;
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1062,8 +1106,12 @@ hipError_t wave_render_views(Context& c, const std::vector<ViewDesc>& views_in, 
             if (it > max_iters) return hipErrorUnknown;
         }
         c.timer.begin(c.stream, "wave_finish");
-        hipLaunchKernelGGL(k_finish, dim3((uint32_t)((pixels + 255) / 256)), dim3(256), 0, c.stream, B.P, films, g.first,
-                           g.second, (uint32_t)pixels);
+        if (nv == 1)
+            hipLaunchKernelGGL(k_finish_one, dim3((uint32_t)((pixels + kFinishBlock - 1) / kFinishBlock)),
+                               dim3(kFinishBlock), 0, c.stream, B.P, films, g.first, g.second, (uint32_t)pixels);
+        else
+            hipLaunchKernelGGL(k_finish, dim3((uint32_t)((pixels + 255) / 256)), dim3(256), 0, c.stream, B.P, films,
+                               g.first, g.second, (uint32_t)pixels);
         c.timer.end(c.stream);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

@@ -154,6 +154,25 @@ def test_wavefront_counters_equal_megakernel(cornell_small, capacity):
     assert 0 < sb["extension_hits"] < sb["extension_rays"]
 
 
+@pytest.mark.parametrize("capacity", [0, 1024])
+def test_wavefront_pixel_major_items_odd_sizes(capacity):
+    """With one view the items of a group run pixel by pixel (a pixel's
+    samples are consecutive items), and the finish kernel reads them back in
+    blocks of 256 pixels and 8 layers: a frame of 37 x 29 pixels (no multiple
+    of the block) at 11 spp (no multiple of 8), continuing a 2-spp render,
+    equals the megakernel, with the pool holding every item or far fewer."""
+    wl = scenes.cornell(subdiv=3, width=37, height=29)
+    a = Renderer.from_workload(wl)
+    a.render(spp=2, frame=1)
+    a.render(spp=11)
+    b = Renderer.from_workload(wl)
+    b.set_wavefront_params(queue_capacity=capacity)
+    b.render_wavefront(spp=2, frame=1)
+    b.render_wavefront(spp=11)
+    assert a.frame == b.frame == 14
+    np.testing.assert_array_equal(b.film(), a.film())
+
+
 def test_wavefront_small_queue_chunks(cornell_small):
     """Queue capacity below the pixel count: the frame is rendered in chunks."""
     wl = cornell_small
