@@ -43,7 +43,7 @@ constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 // iterations per graph replay (even: each replay starts at parity 0); 6
 // measured 1.6% faster than 8 on the 1080p terrain: the drain starts sooner
 constexpr int kIters = 6;
-constexpr uint64_t kMaxGroupItems = 1ull << 27;  // sample-buffer entries per group of layers
+constexpr uint64_t kMaxGroupItems = 1ull << 28;  // sample-buffer entries per group of layers (4.3 GB)
 constexpr int kRefill = 32;  // idle lanes of a wave that trigger a refill from the ray queue
 constexpr uint32_t kDrainMax = 1u << 20;  // paths in flight at which k_drain finishes a group
 constexpr uint32_t kMaxAutoPool = 1u << 25;  // path slots of an automatic pool, at most
