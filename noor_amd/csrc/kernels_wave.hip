@@ -16,7 +16,8 @@
 //   k_shade   walks the class queues in class order, so each wave shades one
 //             material; runs shade_bounce(), the per-bounce function of the
 //             megakernel: next rays -> ray queue, NEE rays -> shadow queue,
-//             ended paths -> free queue
+//             ended paths flagged
+//   k_collect_free lists the flagged slots          -> free queue (slot order)
 //   k_shadow  any hit for the NEE rays (persistent, lane refill), resolved into L
 // Keeping the pool full means every launch has work for the whole GPU (a
 // layer-by-layer wavefront spends most of its time in nearly empty launches
@@ -143,6 +144,9 @@ struct WavePool {
     float4* qo[2] = {nullptr, nullptr};
     float4* qd[2] = {nullptr, nullptr};
     uint32_t* q_free[2] = {nullptr, nullptr};
+    // per slot: its path ended in this iteration's k_shade (k_collect_free
+    // lists the flagged slots on the next free queue in slot order)
+    uint8_t* freed = nullptr;
     uint32_t* q_class = nullptr;  // kClasses * capacity
     uint32_t* q_shadow = nullptr;
     float4* sb = nullptr;         // final radiance per item of the current group
@@ -169,22 +173,21 @@ struct WaveBuffers {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
 
-// Block-aggregated appends to three queues at once (one atomic per queue per
+// Block-aggregated appends to two queues at once (one atomic per queue per
 // block): lane results are this lane's slot in each queue where its flag is
 // set.  Every thread of the block must call it.
-__device__ __forceinline__ void block_append3(uint32_t* c0, bool f0, uint32_t* c1, bool f1, uint32_t* c2, bool f2,
-                                              uint32_t& s0, uint32_t& s1, uint32_t& s2) {
-    __shared__ uint32_t cnt[3][16];
+__device__ __forceinline__ void block_append2(uint32_t* c0, bool f0, uint32_t* c1, bool f1, uint32_t& s0,
+                                              uint32_t& s1) {
+    __shared__ uint32_t cnt[2][16];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1), m2 = __ballot(f2);
+    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
     if (lane == 0) {
         cnt[0][wave] = (uint32_t)__popcll(m0);
         cnt[1][wave] = (uint32_t)__popcll(m1);
-        cnt[2][wave] = (uint32_t)__popcll(m2);
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
-        uint32_t* ctr = threadIdx.x == 0 ? c0 : (threadIdx.x == 1 ? c1 : c2);
+    if (threadIdx.x < 2) {
+        uint32_t* ctr = threadIdx.x == 0 ? c0 : c1;
         uint32_t tot = 0;
         for (uint32_t w = 0; w < nw; ++w) {
             const uint32_t n = cnt[threadIdx.x][w];
@@ -198,7 +201,6 @@ __device__ __forceinline__ void block_append3(uint32_t* c0, bool f0, uint32_t* c
     const unsigned long long below = (1ull << lane) - 1ull;
     s0 = cnt[0][wave] + (uint32_t)__popcll(m0 & below);
     s1 = cnt[1][wave] + (uint32_t)__popcll(m1 & below);
-    s2 = cnt[2][wave] + (uint32_t)__popcll(m2 & below);
 }
 
 // Counter slot of this wave for the ray counts (spread over kStatSlots).
@@ -252,6 +254,7 @@ __global__ void k_stream_init(WavePool B, uint32_t items) {
     if (t < B.capacity) {
         B.q_free[0][t] = t;
         B.item[t] = kNoItem;
+        B.freed[t] = 0;
     }
     if (t == 0) {
         ctl->ray_count[0] = ctl->ray_count[1] = 0;
@@ -580,9 +583,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
             store_state(B, p, ps);
         }
     }
-    uint32_t s1, s2, s3;
-    block_append3(&ctl->shadow_count.v, shadow, &ctl->ray_count[nxt].v, cont, &ctl->free_count[nxt].v, ended, s1, s2,
-                  s3);
+    uint32_t s1, s2;
+    block_append2(&ctl->shadow_count.v, shadow, &ctl->ray_count[nxt].v, cont, s1, s2);
     if (shadow) {
         B.q_shadow[s1] = p;
         B.sh_o[s1] = sho;
@@ -593,7 +595,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEX ? 2 : 4
         B.qo[nxt][s2] = nro;
         B.qd[nxt][s2] = nrd;
     }
-    if (ended) B.q_free[nxt][s3] = p;
+    if (ended) B.freed[p] = 1;
+}
+
+// Lists the slots k_shade flagged on the next free queue, in slot order
+// within each block (one atomic per block): the items k_regen starts next
+// get ascending slots, so their path state is written and read in fewer
+// memory lines than in the order the paths ended (`k_shade` 9% and `k_regen`
+// 5% faster on the 1080p terrain).  A thread takes four consecutive slots'
+// flags in one 32-bit load; the capacity is a multiple of 256.
+__global__ __launch_bounds__(1024) void k_collect_free(WavePool B, int cur) {
+    WaveCtl* ctl = B.ctl;
+    const int nxt = cur ^ 1;
+    __shared__ uint32_t cnt[16];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t p0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4u;
+    uint32_t* flags = reinterpret_cast<uint32_t*>(B.freed);
+    const uint32_t fl = p0 < B.capacity ? flags[p0 >> 2] : 0u;
+    const bool f0 = (fl & 0xFFu) != 0, f1 = (fl & 0xFF00u) != 0, f2 = (fl & 0xFF0000u) != 0,
+               f3 = (fl & 0xFF000000u) != 0;
+    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1), m2 = __ballot(f2), m3 = __ballot(f3);
+    if (lane == 0)
+        cnt[wave] = (uint32_t)(__popcll(m0) + __popcll(m1) + __popcll(m2) + __popcll(m3));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t n = cnt[w];
+            cnt[w] = tot;
+            tot += n;
+        }
+        const uint32_t base = tot ? atomicAdd(&ctl->free_count[nxt].v, tot) : 0u;
+        for (uint32_t w = 0; w < nw; ++w) cnt[w] += base;
+    }
+    __syncthreads();
+    if (fl == 0u) return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t pos = cnt[wave] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) +
+                                          __popcll(m3 & below));
+    uint32_t* q = B.q_free[nxt];
+    if (f0) q[pos++] = p0;
+    if (f1) q[pos++] = p0 + 1;
+    if (f2) q[pos++] = p0 + 2;
+    if (f3) q[pos] = p0 + 3;
+    flags[p0 >> 2] = 0u;
 }
 
 // MIS: a task may hold a second, BSDF-sampled shadow ray (separate
@@ -845,6 +890,7 @@ static hipError_t wave_alloc(WaveBuffers& B, uint32_t cap, uint32_t sb_items, ui
     WA(P.qd[1], cap);
     WA(P.q_free[0], cap);
     WA(P.q_free[1], cap);
+    WA(P.freed, cap);
     WA(P.q_class, (size_t)kClasses * cap);
     WA(P.q_shadow, cap);
     WA(P.sb, sb_items);
@@ -920,6 +966,9 @@ static hipError_t record_iterations(Context& c, WaveBuffers& B, int iters) {
         T.end(c.stream);
         T.begin(c.stream, "wave_shade");
         hipLaunchKernelGGL(shade, g1, b1, 0, c.stream, c.d_scene, B.P, cur);
+        T.end(c.stream);
+        T.begin(c.stream, "wave_collect");
+        hipLaunchKernelGGL(k_collect_free, dim3((cap / 4 + 1023) / 1024), dim3(1024), 0, c.stream, B.P, cur);
         T.end(c.stream);
         T.begin(c.stream, "wave_shadow");
         hipLaunchKernelGGL(shadow, gs, bp, lds, c.stream, c.d_scene, B.P, c.d_stats);
