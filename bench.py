@@ -601,7 +601,8 @@ def main():
                 "rays_per_launch": int((ext_k if dom == "wave_extend" else shd_k) / launches),
                 "limiter": ({"valu_rate": kprof["valu_rate"], "lanes_active": kprof["lanes_active"],
                              "wave_wait_fraction": kprof["wave_wait_fraction"],
-                             "what": "latency of the traversal's dependent fetches with a third of the lanes active "
+                             "what": f"latency of the traversal's dependent fetches with "
+                                     f"{100 * kprof['lanes_active']:.0f}% of the lanes active "
                                      "(hardware counters, see traffic source)"}
                             if kprof else None),
                 "traffic_source": (os.path.relpath(PMC_PROFILE, ROOT) if kprof else None)}
